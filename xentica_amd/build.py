@@ -1,0 +1,61 @@
+"""
+Build ``libxentica_hip.so`` in-tree with hipcc for gfx950.
+
+Run ``python -m xentica_amd.build`` (or ``__graft_entry__.build()``).
+Every HIP translation unit under ``csrc/`` goes into the one library; the
+flags disable packed FP32 instructions and FP contraction.
+"""
+import glob
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+FLAGS = [
+    "-O3", "--offload-arch=%s" % ARCH, "-std=c++17", "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def sources():
+    """All translation units of the library."""
+    files = sorted(glob.glob(os.path.join(HERE, "csrc", "*.cpp")))
+    files += sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
+    return files
+
+
+def build(verbose=False):
+    """Compile every unit and link ``xentica_amd/libxentica_hip.so``."""
+    out_dir = os.path.join(ROOT, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    objects = []
+    for src in sources():
+        obj = os.path.join(out_dir, os.path.basename(src) + ".o")
+        cmd = [HIPCC] + FLAGS + [
+            "-I", os.path.join(ROOT, "include"),
+            "-I", os.path.join(HERE, "csrc"),
+            "-c", src, "-o", obj]
+        if src.endswith(".hip"):
+            cmd[1:1] = ["-x", "hip"]
+        if verbose:
+            print(" ".join(cmd))
+        subprocess.run(cmd, check=True)
+        objects.append(obj)
+    target = os.path.join(HERE, "libxentica_hip.so")
+    tmp = target + ".tmp%d" % os.getpid()
+    cmd = [HIPCC, "--offload-arch=%s" % ARCH, "-shared", "-fPIC", "-o", tmp]
+    cmd += objects + ["-lhiprtc"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, target)
+    return target
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

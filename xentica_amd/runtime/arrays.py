@@ -1,0 +1,85 @@
+"""
+Device arrays with the part of PyCUDA's ``GPUArray`` surface the reference
+uses: ``.get()`` returning a NumPy copy, ``.shape``, ``.dtype``, ``.size``
+(``xentica/core/base.py:544-569, 753, 759-760``; every reference test
+reads ``model.gpu.arrays.cells.get()``).
+
+An array stays on the host until a kernel first needs its device address,
+so models can be built, seeded and inspected on machines without a GPU.
+An owner may attach a ``before_read`` hook that runs ahead of every
+``.get()``; the engines use it to finish deferred steps first.
+"""
+import numpy as np
+
+from xentica_amd.runtime import hip
+
+__all__ = ['DeviceArray', 'to_gpu', 'zeros', ]
+
+
+class DeviceArray:
+    """A flat typed array living in HBM."""
+
+    def __init__(self, host):
+        host = np.ascontiguousarray(host)
+        self.shape = host.shape
+        self.dtype = host.dtype
+        self._host = host.copy()
+        self._buffer = None
+        self.before_read = None
+
+    @property
+    def size(self):
+        """Number of elements."""
+        return int(np.prod(self.shape, dtype=np.int64))
+
+    @property
+    def nbytes(self):
+        """Size in bytes."""
+        return self.size * self.dtype.itemsize
+
+    @property
+    def on_device(self):
+        """``True`` once the data lives in device memory."""
+        return self._buffer is not None
+
+    def device_ptr(self):
+        """Device address, uploading the host copy on first use."""
+        if self._buffer is None:
+            buf = hip.Buffer(self.nbytes)
+            buf.upload(self._host)
+            self._buffer = buf
+            self._host = None
+        return self._buffer.ptr
+
+    def get(self):
+        """Return a NumPy copy of the contents."""
+        if self.before_read is not None:
+            self.before_read()
+        if self._buffer is None:
+            return self._host.copy()
+        out = np.empty(self.shape, dtype=self.dtype)
+        self._buffer.download(out)
+        return out
+
+    def set(self, host):
+        """Replace the contents with ``host`` (same size and dtype)."""
+        host = np.ascontiguousarray(host, dtype=self.dtype).reshape(self.shape)
+        if self._buffer is None:
+            self._host = host.copy()
+        else:
+            self._buffer.upload(host)
+
+    def __repr__(self):
+        where = "device" if self.on_device else "host"
+        return "DeviceArray(shape=%s, dtype=%s, %s)" % (
+            self.shape, self.dtype, where)
+
+
+def to_gpu(host):
+    """Make a :class:`DeviceArray` holding a copy of ``host``."""
+    return DeviceArray(host)
+
+
+def zeros(shape, dtype):
+    """Make a zero-filled :class:`DeviceArray`."""
+    return DeviceArray(np.zeros(shape, dtype=dtype))

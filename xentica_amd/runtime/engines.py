@@ -1,0 +1,84 @@
+"""
+Step engines: how a model's step is executed on the GPU.
+
+* ``plain`` -- the reference's structure (``xentica/core/base.py:703-728``):
+  one ``emit`` launch writing every buffer plane to HBM, then one
+  ``absorb`` launch reading them, per step.  Works for every model the DSL
+  can express and is the parity bridge for the faster engines.
+
+Engines share one interface: ``step()`` (advance one step, possibly
+deferred), ``flush()`` (finish deferred steps), ``load_state(cells,
+colors)`` and ``launch_elementwise(kernel, args, n)``.
+"""
+import numpy as np
+
+from xentica_amd.core.exceptions import XenticaException
+
+__all__ = ['create', 'PlainEngine', 'ENGINES', 'elementwise_grid']
+
+#: Threads per workgroup of the grid-stride kernels (4 wave64s).
+BLOCK = 256
+#: Upper bound on workgroups of a grid-stride launch: 64 per CU on 256 CUs.
+MAX_GRID = 256 * 64
+
+
+def elementwise_grid(count):
+    """Workgroup count of a grid-stride launch over ``count`` items."""
+    return max(1, min((count + BLOCK - 1) // BLOCK, MAX_GRID))
+
+
+class PlainEngine:
+    """Reference-structured emit/absorb launches with buffers in HBM."""
+
+    name = "plain"
+
+    def __init__(self, model):
+        self.model = model
+        if model.cells_num * (len(model.buffers) + 1) >= 2 ** 62:
+            raise XenticaException("field too large for the plain engine")
+        if model.cells_num >= 2 ** 32:
+            raise XenticaException(
+                "the plain engine indexes cells with 32-bit unsigned ints")
+
+    def launch_elementwise(self, kernel, args, count):
+        """Launch a grid-stride kernel over ``count`` items."""
+        kernel(*args, grid=(elementwise_grid(count), 1, 1),
+               block=(BLOCK, 1, 1))
+
+    def step(self):
+        """Launch ``emit`` then ``absorb`` with current parameter values."""
+        model = self.model
+        gpu = model.gpu
+        kernels = gpu.kernels
+        cells_num = np.int64(model.cells_num)
+        args = [param.dtype(param.value) for param in model.emit_params]
+        self.launch_elementwise(
+            kernels.emit, [gpu.arrays.cells] + args + [cells_num],
+            model.cells_num)
+        args = [param.dtype(param.value) for param in model.absorb_params]
+        self.launch_elementwise(
+            kernels.absorb,
+            [gpu.arrays.cells, gpu.arrays.colors] + args + [cells_num],
+            model.cells_num)
+
+    def flush(self):
+        """Nothing is deferred: launches are already queued in order."""
+
+    def load_state(self, cells, colors):
+        """Replace the device arrays' contents."""
+        arrays = self.model.gpu.arrays
+        arrays.cells.set(cells)
+        arrays.colors.set(colors)
+
+
+ENGINES = {
+    'plain': PlainEngine,
+}
+
+
+def create(model, name):
+    """Build the engine ``name`` for ``model``."""
+    if name not in ENGINES:
+        raise XenticaException("unknown engine %r (known: %s)" % (
+            name, ", ".join(sorted(ENGINES))))
+    return ENGINES[name](model)
