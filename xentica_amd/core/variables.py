@@ -102,7 +102,8 @@ class Variable(DeferredExpression, BscaDetectorMixin):
                 for key, val in frame.f_locals.items():
                     if isinstance(val, CellularAutomaton):
                         model = val
-                    if val is self and key not in _BAD_NAMES:
+                    if (val is self and key not in _BAD_NAMES
+                            and key.isidentifier()):
                         return key
         finally:
             del frame
