@@ -1,0 +1,68 @@
+/*
+ * xentica_kernels.h -- hand-written gfx950 kernels of the fast engines.
+ *
+ * These entry points have no one-to-one counterpart in the reference: it
+ * runs every model through the generated emit/absorb kernels launched
+ * from CellularAutomaton.step() (xentica/core/base.py:703-728), with the
+ * state hashed on the host by binascii.crc32 in its tests
+ * (tests/core/test_base.py:39-41).  Each function below computes the same
+ * results as that path for the models it accepts; the comment of each
+ * names the reference behaviour it reproduces.
+ *
+ * Same conventions as xentica_hip.h: 0 on success, xen_last_error() on
+ * failure; all pointers are device pointers unless stated otherwise.
+ */
+#ifndef XENTICA_KERNELS_H
+#define XENTICA_KERNELS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "xentica_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- bit-packed Life-like engine (2-D Moore torus) ----
+ * Replaces the emit+absorb pair of GameOfLife / LifelikeCA
+ * (examples/game_of_life.py:42-70, 190-197) for 1-bit states:
+ * next = ((born >> count) & 1) | ((sustain >> count) & 1) & state.
+ * State: 1 bit per cell, 32 cells per uint32 word, x0 fastest;
+ * width_words = width / 32.  One launch advances `depth` (1..8) steps;
+ * the rows are split into `strips` balanced strips, one wavefront per
+ * (62-word column tile, strip); strips = 0 picks about four waves per SIMD.
+ * The field must be smaller than 2 GiB. */
+int xen_life_bits_launch(const void *src, void *dst, int width_words, int height,
+                         uint32_t born, uint32_t sustain, int depth, int strips,
+                         xen_stream stream);
+/* Advance `steps` steps ping-ponging between buf_a (holding the state) and
+ * buf_b; *result_in_b tells which buffer holds the result. */
+int xen_life_bits_run(void *buf_a, void *buf_b, int width_words, int height,
+                      uint32_t born, uint32_t sustain, long long steps, int depth,
+                      int strips, int *result_in_b, xen_stream stream);
+
+/* ---- layout conversion between the reference plane and packed bits ----
+ * cells: the reference's main-state plane (one uint8 per cell, value 0/1,
+ * xentica/core/base.py:659-661); bits: 32 cells per uint32, x0 fastest.
+ * n_cells must be a multiple of 32. */
+int xen_bits_pack(const uint8_t *cells, uint32_t *bits, long long n_cells, xen_stream stream);
+int xen_bits_unpack(const uint32_t *bits, uint8_t *cells, long long n_cells, xen_stream stream);
+
+/* ---- device CRC-32 (the binascii.crc32 of tests/core/test_base.py:39-41) ----
+ * Zero-initialised, un-inverted CRC-32 register of every `chunk`-byte
+ * chunk of a byte plane (the last chunk may be shorter) into
+ * partials[ceil(n / chunk)]; the host combines them into crc32 of the
+ * whole plane.  The _bits variant hashes the uint8 expansion (one byte
+ * 0/1 per cell) of a packed plane without materialising it; its chunk
+ * must be a multiple of 32. */
+int xen_crc32_u8_partials(const uint8_t *data, long long n, long long chunk,
+                          uint32_t *partials, xen_stream stream);
+int xen_crc32_bits_partials(const uint32_t *bits, long long n_cells, long long chunk,
+                            uint32_t *partials, xen_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XENTICA_KERNELS_H */

@@ -1,0 +1,132 @@
+"""
+The bit-packed Life-like engine against the oracle.
+
+Small and ragged fields are checked step by step against the CPU oracle
+(every step's CRC-32), over all temporal depths, the fast B3/S23 path and
+runtime rules.  At the full 16384x16384 size the engine is checked against
+the oracle for the first steps and against the plain GPU engine (the
+reference's structure) every 50 steps for 500 steps, through the device
+CRC.
+"""
+import binascii
+import zlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from xentica_amd.runtime import hip, kernels
+from xentica_amd.runtime.lifebits import LifeBitsField
+from xentica_amd.tools.rules import LifeLike
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_hashes(cells, width, height, born, sustain, steps):
+    field = O.Field((width, height), 8, np.uint8, cells)
+    return O.lifelike(field, O.moore_deltas(2), O.moore_rev(2), born, sustain,
+                      steps, colors=False, hashes=True), field
+
+
+def rule_masks(golly):
+    rule = LifeLike.golly2int(golly)
+    return rule & 0x1FF, (rule >> 9) & 0x1FF
+
+
+@pytest.mark.parametrize("width,height", [
+    (32, 3), (32, 40), (64, 64), (96, 17), (2016, 37), (1984, 70),
+    (4096, 150), (128, 1)])
+@pytest.mark.parametrize("depth", [1, 3, 4, 8])
+def test_shapes_and_depths_b3s23(width, height, depth):
+    rng = np.random.default_rng(width * 1000 + height + depth)
+    cells = rng.integers(0, 2, width * height, dtype=np.uint8)
+    steps = 2 * depth + 3
+    want, field = oracle_hashes(cells, width, height, 8, 12, steps)
+    life = LifeBitsField(width, height, 8, 12, depth=1)
+    life.set_cells(cells)
+    for step in range(steps):
+        life.launch(depth=1)
+        assert life.crc32() == want[step], "depth-1 step %d" % (step + 1)
+    life = LifeBitsField(width, height, 8, 12, depth=depth)
+    life.set_cells(cells)
+    life.run(steps)
+    life.synchronize()
+    assert np.array_equal(life.get_cells(), field.cells)
+
+
+@pytest.mark.parametrize("golly", [
+    "B3/S23", "B36/S23", "B35678/S5678", "B2/S", "B0/S8", "B012345678/S",
+    "B/S012345678", "B1357/S1357", "B3678/S34678"])
+def test_rules(golly):
+    born, sustain = rule_masks(golly)
+    width, height = 320, 45
+    rng = np.random.default_rng(7)
+    cells = rng.integers(0, 2, width * height, dtype=np.uint8)
+    want, field = oracle_hashes(cells, width, height, born, sustain, 12)
+    for depth in (1, 5):
+        life = LifeBitsField(width, height, born, sustain, depth=depth)
+        life.set_cells(cells)
+        got = []
+        for _ in range(12 // depth + 1):
+            life.launch(depth=depth)
+        life2 = LifeBitsField(width, height, born, sustain, depth=depth)
+        life2.set_cells(cells)
+        life2.run(12)
+        life2.synchronize()
+        assert life2.crc32() == want[11], golly
+    life = LifeBitsField(width, height, born, sustain, depth=1)
+    life.set_cells(cells)
+    for step in range(12):
+        life.launch(depth=1)
+        got.append(life.crc32())
+    assert got == list(want)
+
+
+def test_pack_unpack_and_device_crc():
+    rng = np.random.default_rng(3)
+    n = 32 * 12345
+    cells = rng.integers(0, 2, n, dtype=np.uint8)
+    life = LifeBitsField(32 * 12345, 1)
+    life.set_cells(cells)
+    assert np.array_equal(life.get_cells(), cells)
+    assert life.crc32() == zlib.crc32(cells.tobytes())
+    bits = np.packbits(cells.reshape(-1, 8)[:, ::-1], axis=1).reshape(-1)
+    assert np.array_equal(life.get_bits().view(np.uint8), bits)
+    data = rng.integers(0, 256, 1000003, dtype=np.uint8)
+    buf = hip.Buffer(data.size)
+    buf.upload(data)
+    assert kernels.crc32_u8(buf.ptr, data.size) == binascii.crc32(data)
+
+
+def test_life16k_first_steps_vs_oracle():
+    """16384x16384, seed 42: the first 2 steps equal the oracle's."""
+    from examples.workloads import Life16kExperiment
+    from xentica_amd.seeds.random import LocalRandom
+    size = Life16kExperiment.size
+    n = size[0] * size[1]
+    cells = LocalRandom(Life16kExperiment.word).numpy.randint(0, 2, n)
+    cells = cells.astype(np.uint8)
+    life = LifeBitsField(size[0], size[1], 8, 12, depth=2)
+    life.set_cells(cells)
+    want, field = oracle_hashes(cells, size[0], size[1], 8, 12, 2)
+    life.run(2)
+    life.synchronize()
+    assert life.crc32() == want[1]
+
+
+def test_life16k_vs_plain_engine():
+    """500 steps at 16384x16384: bit-packed and plain engines agree."""
+    from examples.game_of_life import GameOfLife
+    from examples.workloads import Life16kExperiment
+    model = GameOfLife(Life16kExperiment)
+    cells = model.gpu.arrays.cells.get()[:model.cells_num]
+    life = LifeBitsField(16384, 16384, 8, 12, depth=4)
+    life.set_cells(cells)
+    for block in range(10):
+        for _ in range(50):
+            model.step()
+        life.run(50)
+        life.synchronize()
+        plain = kernels.crc32_u8(model.gpu.arrays.cells.device_ptr(),
+                                 model.cells_num)
+        assert life.crc32() == plain, "after %d steps" % (50 * (block + 1))

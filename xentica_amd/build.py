@@ -9,6 +9,7 @@ import glob
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -33,7 +34,7 @@ def build(verbose=False):
     """Compile every unit and link ``xentica_amd/libxentica_hip.so``."""
     out_dir = os.path.join(ROOT, "_build")
     os.makedirs(out_dir, exist_ok=True)
-    objects = []
+    jobs = []
     for src in sources():
         obj = os.path.join(out_dir, os.path.basename(src) + ".o")
         cmd = [HIPCC] + FLAGS + [
@@ -44,8 +45,15 @@ def build(verbose=False):
             cmd[1:1] = ["-x", "hip"]
         if verbose:
             print(" ".join(cmd))
-        subprocess.run(cmd, check=True)
-        objects.append(obj)
+        jobs.append((cmd, obj))
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for proc in pool.map(lambda job: subprocess.run(
+                job[0], capture_output=True, text=True), jobs):
+            if proc.returncode:
+                sys.stderr.write(proc.stdout + proc.stderr)
+                raise subprocess.CalledProcessError(proc.returncode, proc.args)
+    objects = [obj for _, obj in jobs]
     target = os.path.join(HERE, "libxentica_hip.so")
     tmp = target + ".tmp%d" % os.getpid()
     cmd = [HIPCC, "--offload-arch=%s" % ARCH, "-shared", "-fPIC", "-o", tmp]

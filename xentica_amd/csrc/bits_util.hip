@@ -1,0 +1,153 @@
+// Layout conversion (uint8 plane <-> packed bits) and chunked device CRC-32.
+//
+// The CRC is the reflected CRC-32 of zlib/binascii (polynomial 0xEDB88320).
+// Each thread folds one chunk through a 256-entry table held in LDS with a
+// zero initial register; the host combines the chunk registers with the
+// GF(2) shift operator of the chunk lengths (the method of zlib's
+// crc32_combine), then applies the initial value and final inversion.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "xentica_hip.h"
+#include "xentica_kernels.h"
+#include "common.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ __launch_bounds__(kBlock) void pack_kernel(const uint8_t *__restrict__ cells,
+                                                      uint32_t *__restrict__ bits,
+                                                      long long n_words) {
+    const long long w = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (w >= n_words) return;
+    const uint4 *src = reinterpret_cast<const uint4 *>(cells + w * 32);
+    uint32_t word = 0;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const uint4 v = src[q];
+        const uint32_t parts[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                word |= ((parts[p] >> (8 * b)) & 1u) << (q * 16 + p * 4 + b);
+    }
+    bits[w] = word;
+}
+
+__global__ __launch_bounds__(kBlock) void unpack_kernel(const uint32_t *__restrict__ bits,
+                                                        uint8_t *__restrict__ cells,
+                                                        long long n_words) {
+    const long long w = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (w >= n_words) return;
+    const uint32_t word = bits[w];
+    uint4 *dst = reinterpret_cast<uint4 *>(cells + w * 32);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        uint32_t parts[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) v |= ((word >> (q * 16 + p * 4 + b)) & 1u) << (8 * b);
+            parts[p] = v;
+        }
+        dst[q] = make_uint4(parts[0], parts[1], parts[2], parts[3]);
+    }
+}
+
+__device__ void load_crc_table(uint32_t *table) {
+    for (int b = threadIdx.x; b < 256; b += blockDim.x) {
+        uint32_t c = static_cast<uint32_t>(b);
+        for (int k = 0; k < 8; ++k) c = (c & 1u) ? (c >> 1) ^ 0xEDB88320u : c >> 1;
+        table[b] = c;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void crc_u8_kernel(const uint8_t *__restrict__ data,
+                                                        long long n, long long chunk,
+                                                        long long n_chunks,
+                                                        uint32_t *__restrict__ partials) {
+    __shared__ uint32_t table[256];
+    load_crc_table(table);
+    const long long c = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_chunks) return;
+    const long long lo = c * chunk;
+    const long long hi = lo + chunk < n ? lo + chunk : n;
+    uint32_t crc = 0;
+    for (long long i = lo; i < hi; ++i) crc = table[(crc ^ data[i]) & 0xFFu] ^ (crc >> 8);
+    partials[c] = crc;
+}
+
+__global__ __launch_bounds__(kBlock) void crc_bits_kernel(const uint32_t *__restrict__ bits,
+                                                          long long n_cells, long long chunk,
+                                                          long long n_chunks,
+                                                          uint32_t *__restrict__ partials) {
+    __shared__ uint32_t table[256];
+    load_crc_table(table);
+    const long long c = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n_chunks) return;
+    const long long lo = c * chunk;
+    const long long hi = lo + chunk < n_cells ? lo + chunk : n_cells;
+    uint32_t crc = 0;
+    for (long long w = lo / 32; w * 32 < hi; ++w) {
+        const uint32_t word = bits[w];
+        const int count = hi - w * 32 < 32 ? static_cast<int>(hi - w * 32) : 32;
+        for (int b = 0; b < count; ++b)
+            crc = table[(crc ^ ((word >> b) & 1u)) & 0xFFu] ^ (crc >> 8);
+    }
+    partials[c] = crc;
+}
+
+unsigned groups_for(long long items) {
+    return static_cast<unsigned>((items + kBlock - 1) / kBlock);
+}
+
+}  // namespace
+
+extern "C" {
+
+int xen_bits_pack(const uint8_t *cells, uint32_t *bits, long long n_cells, xen_stream stream) {
+    if (n_cells % 32) return xen::fail("xen_bits_pack: n_cells must be a multiple of 32", 22);
+    const long long words = n_cells / 32;
+    if (!words) return 0;
+    hipLaunchKernelGGL(pack_kernel, dim3(groups_for(words)), dim3(kBlock), 0,
+                       reinterpret_cast<hipStream_t>(stream), cells, bits, words);
+    return xen::check(hipGetLastError(), "pack_kernel launch");
+}
+
+int xen_bits_unpack(const uint32_t *bits, uint8_t *cells, long long n_cells, xen_stream stream) {
+    if (n_cells % 32) return xen::fail("xen_bits_unpack: n_cells must be a multiple of 32", 22);
+    const long long words = n_cells / 32;
+    if (!words) return 0;
+    hipLaunchKernelGGL(unpack_kernel, dim3(groups_for(words)), dim3(kBlock), 0,
+                       reinterpret_cast<hipStream_t>(stream), bits, cells, words);
+    return xen::check(hipGetLastError(), "unpack_kernel launch");
+}
+
+int xen_crc32_u8_partials(const uint8_t *data, long long n, long long chunk, uint32_t *partials,
+                          xen_stream stream) {
+    if (chunk < 1) return xen::fail("xen_crc32_u8_partials: chunk must be positive", 22);
+    const long long n_chunks = (n + chunk - 1) / chunk;
+    if (!n_chunks) return 0;
+    hipLaunchKernelGGL(crc_u8_kernel, dim3(groups_for(n_chunks)), dim3(kBlock), 0,
+                       reinterpret_cast<hipStream_t>(stream), data, n, chunk, n_chunks,
+                       partials);
+    return xen::check(hipGetLastError(), "crc_u8_kernel launch");
+}
+
+int xen_crc32_bits_partials(const uint32_t *bits, long long n_cells, long long chunk,
+                            uint32_t *partials, xen_stream stream) {
+    if (chunk < 32 || chunk % 32)
+        return xen::fail("xen_crc32_bits_partials: chunk must be a positive multiple of 32", 22);
+    const long long n_chunks = (n_cells + chunk - 1) / chunk;
+    if (!n_chunks) return 0;
+    hipLaunchKernelGGL(crc_bits_kernel, dim3(groups_for(n_chunks)), dim3(kBlock), 0,
+                       reinterpret_cast<hipStream_t>(stream), bits, n_cells, chunk, n_chunks,
+                       partials);
+    return xen::check(hipGetLastError(), "crc_bits_kernel launch");
+}
+
+}  // extern "C"

@@ -1,0 +1,355 @@
+// Bit-packed Life-like engine for 2-D Moore tori (gfx950, wave64).
+//
+// State layout: one bit per cell, 32 cells per 32-bit word, x0 fastest
+// (cell x of row y is bit x%32 of word y*W + x/32), which is byte-for-byte
+// the layout of 64 cells per little-endian uint64.  The step is the
+// reference's Life-like rule (examples/game_of_life.py:55-70, 190-197):
+// born = (B >> count) & 1, sustained = (S >> count) & 1,
+// next = born | sustained & state, count = live cells among the eight
+// Moore neighbours on a torus (xentica/core/topology/border.py:104-117).
+//
+// Kernel structure (one wavefront per tile, no LDS, no barriers):
+//   * lane l owns one word column; the wave spans 64 adjacent columns of
+//     which the 62 inner ones are output (lanes 0 and 63 are halo: their
+//     results go stale one bit per step from the outer edge, so up to 31
+//     steps never reach an inner lane);
+//   * each lane keeps R = OUT + 2K rows of its column in VGPRs and applies
+//     K steps in registers (temporal blocking: K steps per HBM round trip);
+//   * vertical neighbour sums come from registers, horizontal ones from the
+//     adjacent lanes through DPP wave shifts plus a funnel shift;
+//   * the neighbour count is a bit-sliced carry-save adder over 32 cells at
+//     once; for B3/S23 the rule collapses to two 3-input logic ops.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "xentica_hip.h"
+#include "xentica_kernels.h"
+#include "common.h"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kOutLanes = kWave - 2;
+constexpr int kWavesPerGroup = 4;
+// dword 3 of a raw buffer resource on gfx9-family (CDNA) parts
+constexpr int kRsrcFlags = 0x00020000;
+// keep several waves per SIMD resident (caps VGPRs at 512 / this)
+constexpr int kMinWavesPerSimd = 4;
+
+// Value of lane (i - 1) in lane i (DPP wave_shr:1).
+__device__ __forceinline__ uint32_t from_left_lane(uint32_t v) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x138, 0xF, 0xF, true));
+}
+
+// Value of lane (i + 1) in lane i (DPP wave_shl:1).
+__device__ __forceinline__ uint32_t from_right_lane(uint32_t v) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x130, 0xF, 0xF, true));
+}
+
+// v_bitop3_b32: result bit = imm[(a << 2) | (b << 1) | c], i.e. the
+// immediate is the function evaluated on a = 0xF0, b = 0xCC, c = 0xAA
+// (the convention hipcc itself uses when it selects the instruction).
+template <uint8_t kTable>
+__device__ __forceinline__ uint32_t bitop3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, kTable);
+}
+constexpr uint8_t kXor3 = 0x96;  // a ^ b ^ c
+constexpr uint8_t kMaj3 = 0xE8;  // at least two of a, b, c
+constexpr uint8_t kMux = 0xCA;   // a ? b : c
+constexpr uint8_t kXorAndNot = 0x14;  // (a ^ b) & ~c
+constexpr uint8_t kAndOr = 0xE0;      // a & (b | c)
+
+// Rule application.  The neighbour count is count = c0 + 2 * s2 with
+// s2 = x + k0 + 2y (x, k0, y single bits); kFast selects B3/S23, where
+// next = (s2 == 1) & (c0 | m) = ((x ^ k0) & ~y) & (c0 | m).  Otherwise the
+// rule masks are runtime values and the lookup is a mux tree over the
+// count bits with the state muxed in at the leaves.
+template <bool kFast>
+struct Rule;
+
+template <>
+struct Rule<true> {
+    uint32_t born, sustain;
+    __device__ __forceinline__ uint32_t apply(uint32_t c0, uint32_t x, uint32_t k0, uint32_t y,
+                                              uint32_t m) const {
+        return bitop3<kAndOr>(bitop3<kXorAndNot>(x, k0, y), c0, m);
+    }
+};
+
+template <>
+struct Rule<false> {
+    uint32_t born, sustain;
+    __device__ __forceinline__ uint32_t apply(uint32_t c0, uint32_t x, uint32_t k0, uint32_t y,
+                                              uint32_t m) const {
+        const uint32_t c1 = x ^ k0;
+        const uint32_t k1 = x & k0;
+        const uint32_t c2 = y ^ k1;
+        const uint32_t c3 = y & k1;
+        uint32_t leaf[9];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+            const uint32_t b = ((born >> v) & 1u) ? 0xFFFFFFFFu : 0u;
+            const uint32_t s = ((sustain >> v) & 1u) ? 0xFFFFFFFFu : 0u;
+            leaf[v] = bitop3<kMux>(m, s, b);
+        }
+        const uint32_t l0 = bitop3<kMux>(c0, leaf[1], leaf[0]);
+        const uint32_t l1 = bitop3<kMux>(c0, leaf[3], leaf[2]);
+        const uint32_t l2 = bitop3<kMux>(c0, leaf[5], leaf[4]);
+        const uint32_t l3 = bitop3<kMux>(c0, leaf[7], leaf[6]);
+        const uint32_t m0 = bitop3<kMux>(c1, l1, l0);
+        const uint32_t m1 = bitop3<kMux>(c1, l3, l2);
+        const uint32_t n0 = bitop3<kMux>(c2, m1, m0);
+        return bitop3<kMux>(c3, leaf[8], n0);
+    }
+};
+
+// New state of one word (32 cells) from its column's up/mid/down words.
+template <bool kFast>
+__device__ __forceinline__ uint32_t life_word(uint32_t u, uint32_t m, uint32_t d,
+                                              const Rule<kFast> &rule) {
+    // vertical sums: b = up + down (the cell's own column, without the
+    // cell), a = up + mid + down (the side columns); two bits each
+    const uint32_t b0 = u ^ d;
+    const uint32_t b1 = u & d;
+    const uint32_t a0 = b0 ^ m;
+    const uint32_t a1 = (m & b0) | b1;
+    // column sums of the left and right neighbour columns
+    const uint32_t a0l = from_left_lane(a0), a1l = from_left_lane(a1);
+    const uint32_t a0r = from_right_lane(a0), a1r = from_right_lane(a1);
+    const uint32_t l0 = (a0 << 1) | (a0l >> 31), l1 = (a1 << 1) | (a1l >> 31);
+    const uint32_t r0 = (a0 >> 1) | (a0r << 31), r1 = (a1 >> 1) | (a1r << 31);
+    // count = (l0 + 2 l1) + (r0 + 2 r1) + (b0 + 2 b1), bit-sliced:
+    // c0 + 2 k0 = l0 + r0 + b0 and x + 2 y = l1 + r1 + b1
+    const uint32_t c0 = bitop3<kXor3>(l0, r0, b0);
+    const uint32_t k0 = bitop3<kMaj3>(l0, r0, b0);
+    const uint32_t x = bitop3<kXor3>(l1, r1, b1);
+    const uint32_t y = bitop3<kMaj3>(l1, r1, b1);
+    return rule.apply(c0, x, k0, y, m);
+}
+
+__device__ __forceinline__ int wrap(int v, int n) {
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+// Pipeline state: lev[l][j % 3] is row j of step level l (0 = input).
+template <int K>
+struct Window {
+    uint32_t lev[K + 1][3];
+};
+
+// Feed input row i (with i % 3 == I3) into the pipeline and advance every
+// level whose row is valid: level l produces row i - l when i >= 2l.
+// kAll asserts i >= 2K (steady state, every level active).
+template <int K, bool kFast, int I3, bool kAll>
+__device__ __forceinline__ void feed(Window<K> &w, uint32_t input, int i,
+                                     const Rule<kFast> &rule) {
+    w.lev[0][I3] = input;
+#pragma unroll
+    for (int l = 1; l <= K; ++l) {
+        if (kAll || i >= 2 * l) {
+            const int j3 = ((I3 - l) % 3 + 3) % 3;
+            w.lev[l][j3] = life_word<kFast>(w.lev[l - 1][(j3 + 2) % 3], w.lev[l - 1][j3],
+                                            w.lev[l - 1][(j3 + 1) % 3], rule);
+        }
+    }
+}
+
+template <int K, int I3>
+__device__ __forceinline__ uint32_t level_k_row(const Window<K> &w) {
+    return w.lev[K][((I3 - K) % 3 + 3) % 3];
+}
+
+// One launch = K steps over the whole field.  Wave w owns tile (tx, ty):
+// output word columns [col_lo, col_hi) (at most 62) and output rows
+// [row_lo, row_hi); both splits are balanced (sizes differ by at most
+// one).  It streams the L = out + 2K rows around them
+// through a K-level row pipeline: when input row i arrives, level l
+// computes row i - l from rows i-l-1 .. i-l+1 of level l-1, so level l
+// holds valid rows [l, L-1-l] and only three rows per level are live.
+template <int K, bool kFast>
+__global__ __launch_bounds__(kWave * kWavesPerGroup)
+__attribute__((amdgpu_waves_per_eu(kMinWavesPerSimd)))
+void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                      int W, int H, int tiles_x, int tiles_y,
+                      uint32_t born, uint32_t sustain) {
+    const int lane = threadIdx.x & (kWave - 1);
+    // wave-uniform tile coordinates live in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(
+        static_cast<int>(blockIdx.x) * kWavesPerGroup + (static_cast<int>(threadIdx.x) >> 6));
+    if (wave >= tiles_x * tiles_y) return;
+    const int tx = wave % tiles_x;
+    const int ty = wave / tiles_x;
+    // balanced split of the W output columns over tiles_x tiles
+    const int col_lo = static_cast<int>((static_cast<long long>(tx) * W) / tiles_x);
+    const int col_hi = static_cast<int>((static_cast<long long>(tx + 1) * W) / tiles_x);
+    const int col = wrap(col_lo - 1 + lane, W);
+    const int row_lo = static_cast<int>((static_cast<long long>(ty) * H) / tiles_y);
+    const int row_hi = static_cast<int>((static_cast<long long>(ty + 1) * H) / tiles_y);
+    const int out = row_hi - row_lo;
+    const int L = out + 2 * K;
+    const int row0 = row_lo - K;
+
+    // Buffer resources: scalar base and row offsets, one per-lane column
+    // offset.  Out-of-range offsets are dropped by the hardware range
+    // check, which is how masked-off stores are expressed.
+    const unsigned bytes = static_cast<unsigned>(static_cast<size_t>(W) * H * 4u);
+    const __amdgpu_buffer_rsrc_t in_rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(src), 0, bytes, kRsrcFlags);
+    const __amdgpu_buffer_rsrc_t out_rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(dst, 0, bytes, kRsrcFlags);
+    const int row_bytes = W * 4;
+    const int lane_off = col * 4;
+    const int out_col = col_lo - 1 + lane;
+    const bool lane_out = lane >= 1 && out_col < col_hi;
+    const int out_voff = lane_out ? out_col * 4 : static_cast<int>(bytes);
+    const int out_row0 = row_lo;
+    const int out_rows = out;
+
+    auto load_row = [&](int i) -> uint32_t {
+        return __builtin_amdgcn_raw_buffer_load_b32(in_rsrc, lane_off,
+                                                    wrap(row0 + i, H) * row_bytes, 0);
+    };
+    // output row r = i - 2K of the tile, fed at input row i
+    auto store_row = [&](uint32_t val, int r) {
+        const bool keep = r < out_rows;
+        __builtin_amdgcn_raw_buffer_store_b32(val, out_rsrc,
+                                              keep ? out_voff : static_cast<int>(bytes),
+                                              keep ? (out_row0 + r) * row_bytes : 0, 0);
+    };
+
+    Rule<kFast> rule{born, sustain};
+    Window<K> w;
+    // prologue: rows 0 .. 2K-1 fill the pipeline (no output yet)
+#pragma unroll
+    for (int i = 0; i < 2 * K; ++i) {
+        const uint32_t in = load_row(i);
+        switch (i % 3) {
+            case 0: feed<K, kFast, 0, false>(w, in, i, rule); break;
+            case 1: feed<K, kFast, 1, false>(w, in, i, rule); break;
+            default: feed<K, kFast, 2, false>(w, in, i, rule); break;
+        }
+    }
+    // steady state: rows 2K .. L-1, three per iteration so that the
+    // window slots are compile-time constants
+    constexpr int P0 = (2 * K) % 3, P1 = (2 * K + 1) % 3, P2 = (2 * K + 2) % 3;
+    int i = 2 * K;
+    uint32_t a = load_row(i), b = load_row(i + 1), c = load_row(i + 2);
+    for (; i + 2 < L; i += 3) {
+        const uint32_t na = load_row(i + 3), nb = load_row(i + 4), nc = load_row(i + 5);
+        feed<K, kFast, P0, true>(w, a, i, rule);
+        store_row(level_k_row<K, P0>(w), i - 2 * K);
+        feed<K, kFast, P1, true>(w, b, i + 1, rule);
+        store_row(level_k_row<K, P1>(w), i + 1 - 2 * K);
+        feed<K, kFast, P2, true>(w, c, i + 2, rule);
+        store_row(level_k_row<K, P2>(w), i + 2 - 2 * K);
+        a = na;
+        b = nb;
+        c = nc;
+    }
+    if (i < L) {
+        feed<K, kFast, P0, true>(w, a, i, rule);
+        store_row(level_k_row<K, P0>(w), i - 2 * K);
+        if (i + 1 < L) {
+            feed<K, kFast, P1, true>(w, b, i + 1, rule);
+            store_row(level_k_row<K, P1>(w), i + 1 - 2 * K);
+        }
+    }
+}
+
+template <int K, bool kFast>
+int launch_k(const uint32_t *src, uint32_t *dst, int W, int H, int tiles_x, int tiles_y,
+             uint32_t born, uint32_t sustain, hipStream_t stream) {
+    const long long waves = static_cast<long long>(tiles_x) * tiles_y;
+    const unsigned groups = static_cast<unsigned>((waves + kWavesPerGroup - 1) / kWavesPerGroup);
+    hipLaunchKernelGGL((life_bits_kernel<K, kFast>), dim3(groups), dim3(kWave * kWavesPerGroup),
+                       0, stream, src, dst, W, H, tiles_x, tiles_y, born, sustain);
+    return xen::check(hipGetLastError(), "life_bits_kernel launch");
+}
+
+template <bool kFast>
+int launch_any_k(int k, const uint32_t *src, uint32_t *dst, int W, int H, int tiles_x,
+                 int tiles_y, uint32_t born, uint32_t sustain, hipStream_t stream) {
+    switch (k) {
+        case 1: return launch_k<1, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
+        case 2: return launch_k<2, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
+        case 3: return launch_k<3, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
+        case 4: return launch_k<4, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
+        case 5: return launch_k<5, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
+        case 6: return launch_k<6, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
+        case 7: return launch_k<7, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
+        case 8: return launch_k<8, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
+        default: return xen::fail("life_bits: temporal depth must be 1..8", 22);
+    }
+}
+
+// Row strips per launch when the caller passes 0: about four waves per
+// SIMD of the device, and strips at least 4K rows tall.
+int auto_strips(int tiles_x, int H, int depth) {
+    static int simds = 0;
+    if (!simds) {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            simds = 4 * cus;
+        else
+            simds = 1024;
+    }
+    long long strips = (4LL * simds) / tiles_x;
+    const long long max_strips = H / (4 * depth) > 0 ? H / (4 * depth) : 1;
+    if (strips > max_strips) strips = max_strips;
+    return strips < 1 ? 1 : static_cast<int>(strips);
+}
+
+}  // namespace
+
+extern "C" {
+
+int xen_life_bits_launch(const void *src, void *dst, int width_words, int height,
+                         uint32_t born, uint32_t sustain, int depth, int strips,
+                         xen_stream stream) {
+    if (width_words < 1 || height < 1)
+        return xen::fail("xen_life_bits_launch: empty field", 22);
+    if (src == dst) return xen::fail("xen_life_bits_launch: src and dst must differ", 22);
+    if (static_cast<long long>(width_words) * height * 4 >= (1LL << 31))
+        return xen::fail("xen_life_bits_launch: field must be smaller than 2 GiB", 22);
+    if (depth < 1 || depth > 8) return xen::fail("xen_life_bits_launch: depth must be 1..8", 22);
+    const int tiles_x = (width_words + kOutLanes - 1) / kOutLanes;
+    int tiles_y = strips > 0 ? strips : auto_strips(tiles_x, height, depth);
+    if (tiles_y > height) tiles_y = height;
+    auto s = reinterpret_cast<hipStream_t>(stream);
+    auto in = static_cast<const uint32_t *>(src);
+    auto out = static_cast<uint32_t *>(dst);
+    const bool fast = (born & 0x1FFu) == 0x8u && (sustain & 0x1FFu) == 0xCu;
+    if (fast)
+        return launch_any_k<true>(depth, in, out, width_words, height, tiles_x, tiles_y, born,
+                                  sustain, s);
+    return launch_any_k<false>(depth, in, out, width_words, height, tiles_x, tiles_y, born,
+                               sustain, s);
+}
+
+int xen_life_bits_run(void *buf_a, void *buf_b, int width_words, int height,
+                      uint32_t born, uint32_t sustain, long long steps, int depth,
+                      int strips, int *result_in_b, xen_stream stream) {
+    if (depth < 1 || depth > 8) return xen::fail("xen_life_bits_run: depth must be 1..8", 22);
+    void *cur = buf_a, *nxt = buf_b;
+    int in_b = 0;
+    long long left = steps;
+    while (left > 0) {
+        const int k = left >= depth ? depth : static_cast<int>(left);
+        if (int rc = xen_life_bits_launch(cur, nxt, width_words, height, born, sustain, k,
+                                          strips, stream))
+            return rc;
+        void *t = cur;
+        cur = nxt;
+        nxt = t;
+        in_b ^= 1;
+        left -= k;
+    }
+    *result_in_b = in_b;
+    return 0;
+}
+
+}  // extern "C"

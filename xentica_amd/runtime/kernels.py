@@ -1,0 +1,108 @@
+"""
+ctypes binding of the hand-written kernels (``include/xentica_kernels.h``).
+
+Pointers are integer device addresses (``Buffer.ptr`` /
+``DeviceArray.device_ptr()``); streams are :class:`hip.Stream` or ``None``.
+"""
+import ctypes
+
+import numpy as np
+
+from xentica_amd.runtime import hip
+from xentica_amd.runtime.crc import combine_partials
+
+__all__ = [
+    'life_bits_launch', 'life_bits_run', 'bits_pack', 'bits_unpack',
+    'crc32_u8', 'crc32_bits', 'CRC_CHUNK',
+]
+
+_vp = ctypes.c_void_p
+_ll = ctypes.c_longlong
+_u32 = ctypes.c_uint32
+_i = ctypes.c_int
+
+_SIGNATURES = {
+    'xen_life_bits_launch': [_vp, _vp, _i, _i, _u32, _u32, _i, _i, _vp],
+    'xen_life_bits_run': [_vp, _vp, _i, _i, _u32, _u32, _ll, _i, _i,
+                          ctypes.POINTER(_i), _vp],
+    'xen_bits_pack': [_vp, _vp, _ll, _vp],
+    'xen_bits_unpack': [_vp, _vp, _ll, _vp],
+    'xen_crc32_u8_partials': [_vp, _ll, _ll, _vp, _vp],
+    'xen_crc32_bits_partials': [_vp, _ll, _ll, _vp, _vp],
+}
+_BOUND = False
+
+#: Bytes per CRC chunk (one thread each); a multiple of 32.
+CRC_CHUNK = 4096
+
+
+def _lib():
+    global _BOUND  # pylint: disable=global-statement
+    lib = hip.library()
+    if not _BOUND:
+        for name, argtypes in _SIGNATURES.items():
+            func = getattr(lib, name)
+            func.argtypes = argtypes
+            func.restype = ctypes.c_int
+        _BOUND = True
+    return lib
+
+
+def _s(stream):
+    return stream.handle if stream is not None else None
+
+
+def life_bits_launch(src, dst, width_words, height, born, sustain, depth,
+                     strips=0, stream=None):
+    """One launch of ``depth`` Life-like steps on a packed torus."""
+    hip.check(_lib().xen_life_bits_launch(
+        src, dst, width_words, height, born, sustain, depth, strips,
+        _s(stream)), "xen_life_bits_launch")
+
+
+def life_bits_run(buf_a, buf_b, width_words, height, born, sustain, steps,
+                  depth, strips=0, stream=None):
+    """Run ``steps`` steps; returns True when the result is in ``buf_b``."""
+    in_b = _i(0)
+    hip.check(_lib().xen_life_bits_run(
+        buf_a, buf_b, width_words, height, born, sustain, steps, depth,
+        strips, ctypes.byref(in_b), _s(stream)), "xen_life_bits_run")
+    return bool(in_b.value)
+
+
+def bits_pack(cells, bits, n_cells, stream=None):
+    """uint8 plane (0/1 per cell) -> packed bits, on the device."""
+    hip.check(_lib().xen_bits_pack(cells, bits, n_cells, _s(stream)),
+              "xen_bits_pack")
+
+
+def bits_unpack(bits, cells, n_cells, stream=None):
+    """Packed bits -> uint8 plane, on the device."""
+    hip.check(_lib().xen_bits_unpack(bits, cells, n_cells, _s(stream)),
+              "xen_bits_unpack")
+
+
+def _partials_buffer(n_chunks):
+    return hip.Buffer(max(1, n_chunks) * 4)
+
+
+def crc32_u8(data_ptr, nbytes, stream=None, chunk=CRC_CHUNK):
+    """``binascii.crc32`` of ``nbytes`` device bytes."""
+    n_chunks = (nbytes + chunk - 1) // chunk
+    parts = _partials_buffer(n_chunks)
+    hip.check(_lib().xen_crc32_u8_partials(
+        data_ptr, nbytes, chunk, parts.ptr, _s(stream)), "crc32 u8")
+    host = np.empty(n_chunks, dtype=np.uint32)
+    parts.download(host, stream)
+    return combine_partials(host, chunk, nbytes)
+
+
+def crc32_bits(bits_ptr, n_cells, stream=None, chunk=CRC_CHUNK):
+    """``binascii.crc32`` of the uint8 expansion of a packed plane."""
+    n_chunks = (n_cells + chunk - 1) // chunk
+    parts = _partials_buffer(n_chunks)
+    hip.check(_lib().xen_crc32_bits_partials(
+        bits_ptr, n_cells, chunk, parts.ptr, _s(stream)), "crc32 bits")
+    host = np.empty(n_chunks, dtype=np.uint32)
+    parts.download(host, stream)
+    return combine_partials(host, chunk, n_cells)

@@ -1,0 +1,119 @@
+"""
+Bit-packed Life-like fields on the GPU (the fast engine of 2-D Moore tori).
+
+A :class:`LifeBitsField` holds a ``width x height`` torus with one bit per
+cell (32 cells per 32-bit word, x0 fastest -- the same bytes as 64 cells
+per little-endian uint64) in two ping-pong buffers, and advances it with
+``xen_life_bits_run``: ``depth`` steps per launch, kept in registers.  The
+rule is the reference's Life-like rule (``examples/game_of_life.py:55-70,
+190-197``): ``born = (B >> count) & 1``, ``sustained = (S >> count) & 1``,
+``next = born | sustained & state``.
+"""
+import numpy as np
+
+from xentica_amd.core.exceptions import XenticaException
+from xentica_amd.runtime import hip, kernels
+
+__all__ = ['LifeBitsField', ]
+
+
+class LifeBitsField:
+    """
+    A packed Life-like torus resident in HBM.
+
+    :param width: cells per row (a multiple of 32)
+    :param height: rows
+    :param born: birth mask (bit ``c`` set: born with ``c`` neighbours)
+    :param sustain: survival mask
+    :param depth: steps per launch (1..8)
+    :param strips: row strips per launch (0 = automatic)
+    """
+
+    def __init__(self, width, height, born=8, sustain=12, depth=4, strips=0,
+                 stream=None):
+        if width % 32 or width <= 0 or height <= 0:
+            raise XenticaException(
+                "bit-packed fields need a width that is a positive multiple "
+                "of 32 (got %dx%d)" % (width, height))
+        self.width, self.height = int(width), int(height)
+        self.words = self.width // 32
+        self.n_cells = self.width * self.height
+        self.born, self.sustain = int(born), int(sustain)
+        self.depth, self.strips = int(depth), int(strips)
+        self.stream = stream
+        nbytes = self.n_cells // 8
+        self._bufs = [hip.Buffer(nbytes), hip.Buffer(nbytes)]
+        self._cur = 0
+
+    @property
+    def state_ptr(self):
+        """Device address of the buffer holding the current state."""
+        return self._bufs[self._cur].ptr
+
+    @property
+    def nbytes(self):
+        """Bytes of one packed state."""
+        return self.n_cells // 8
+
+    def set_cells(self, cells):
+        """Load a uint8 plane (one 0/1 byte per cell, x0 fastest)."""
+        cells = np.ascontiguousarray(cells, dtype=np.uint8).reshape(-1)
+        if cells.size != self.n_cells:
+            raise XenticaException("expected %d cells, got %d" % (
+                self.n_cells, cells.size))
+        staging = hip.Buffer(self.n_cells)
+        staging.upload(cells & 1, self.stream)
+        kernels.bits_pack(staging.ptr, self.state_ptr, self.n_cells,
+                          self.stream)
+        self.synchronize()
+
+    def set_bits(self, bits):
+        """Load a packed state (``n_cells / 32`` uint32 words)."""
+        bits = np.ascontiguousarray(bits, dtype=np.uint32).reshape(-1)
+        self._bufs[self._cur].upload(bits, self.stream)
+
+    def get_cells(self):
+        """The state as a uint8 plane."""
+        staging = hip.Buffer(self.n_cells)
+        kernels.bits_unpack(self.state_ptr, staging.ptr, self.n_cells,
+                            self.stream)
+        out = np.empty(self.n_cells, dtype=np.uint8)
+        staging.download(out, self.stream)
+        return out
+
+    def get_bits(self):
+        """The packed state as uint32 words."""
+        out = np.empty(self.n_cells // 32, dtype=np.uint32)
+        self._bufs[self._cur].download(out, self.stream)
+        return out
+
+    def crc32(self):
+        """``binascii.crc32`` of the uint8 plane, computed on the device."""
+        return kernels.crc32_bits(self.state_ptr, self.n_cells, self.stream)
+
+    def launch(self, depth=None):
+        """One launch of ``depth`` steps (asynchronous)."""
+        depth = self.depth if depth is None else depth
+        kernels.life_bits_launch(
+            self.state_ptr, self._bufs[1 - self._cur].ptr, self.words,
+            self.height, self.born, self.sustain, depth, self.strips,
+            self.stream)
+        self._cur = 1 - self._cur
+
+    def run(self, steps):
+        """Advance ``steps`` steps (asynchronous)."""
+        if steps <= 0:
+            return
+        in_b = kernels.life_bits_run(
+            self.state_ptr, self._bufs[1 - self._cur].ptr, self.words,
+            self.height, self.born, self.sustain, steps, self.depth,
+            self.strips, self.stream)
+        if in_b:
+            self._cur = 1 - self._cur
+
+    def synchronize(self):
+        """Wait for queued work."""
+        if self.stream is not None:
+            self.stream.synchronize()
+        else:
+            hip.device().synchronize()
