@@ -1,0 +1,212 @@
+"""
+Headline benchmark: Game of Life B3/S23 on a 16384x16384 torus.
+
+One "step" is one generation of the whole field.  The field is seeded as
+the Life experiment with word 42 (``RandInt(0, 1)`` on every cell, 50 %
+density), packed to 64 cells per uint64 in HBM, and advanced by the
+bit-packed engine (``xen_life_bits_run``: temporal depth K steps per
+launch).  Inputs are resident in HBM before the timed region; the timed
+region is exactly ``--steps`` generations bracketed by device
+synchronisation, timed with HIP events on the engine's stream.
+
+The JSON line also carries:
+* ``roofline`` -- algorithmic bytes (one bit read + one bit written per
+  cell-update = 0.25 B) per launch over the average launch time, against
+  MI355X's 8 TB/s HBM peak; ``traffic`` is the HBM bytes per launch from
+  the committed rocprofv3 counter summary when it exists;
+* ``cpu_baseline`` -- the oracle (the reference's plain emit/absorb
+  structure in C, OpenMP over every host core) on a bounded sample of the
+  same workload;
+* ``plain_engine`` -- the reference-structured GPU path (generated
+  emit/absorb kernels with the buffer planes in HBM) on the same field.
+
+Usage: ``python bench.py [--gpus 1] [--steps 2000] [--warmup 200]``.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ("cell-updates/s and % HBM roofline; batched and surrogate speedup "
+          "vs plain")
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_UPDATE = 0.25  # 1 bit read + 1 bit written
+SIZE = (16384, 16384)
+WORD = 42
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "life_bits_16384_traffic.json")
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def seed_cells():
+    """Step-0 cells of the Life experiment (word 42, RandInt(0, 1))."""
+    from xentica_amd.seeds.random import LocalRandom
+    n = SIZE[0] * SIZE[1]
+    return LocalRandom(WORD).numpy.randint(0, 2, n).astype(np.uint8)
+
+
+def timed(field, stream, steps):
+    """Run ``steps`` steps between HIP events on ``stream``; returns ms."""
+    from xentica_amd.runtime import hip
+    start, stop = hip.Event(), hip.Event()
+    stream.synchronize()
+    start.record(stream)
+    field.run(steps)
+    stop.record(stream)
+    stop.synchronize()
+    return start.elapsed_ms(stop)
+
+
+def autotune(field, stream, trial_steps=96):
+    """Pick the temporal depth with the best throughput on this device."""
+    best = None
+    for depth in (2, 3, 4, 5, 6, 8):
+        field.depth = depth
+        timed(field, stream, 2 * depth)
+        ms = timed(field, stream, trial_steps)
+        log("autotune depth=%d: %.3f ms / %d steps" % (depth, ms, trial_steps))
+        if best is None or ms < best[1]:
+            best = (depth, ms)
+    field.depth = best[0]
+    return best[0]
+
+
+def traffic_per_launch():
+    """HBM bytes per launch from the committed counter summary, if any."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as summary:
+        return json.load(summary).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(cells, steps=4):
+    """The oracle on a bounded sample: the same field, ``steps`` steps."""
+    from oracle import oracle
+    cores = len(os.sched_getaffinity(0))
+    field = oracle.Field(SIZE, 8, np.uint8, cells)
+    oracle.lifelike(field, oracle.moore_deltas(2), oracle.moore_rev(2), 8, 12,
+                    1, colors=False, threads=cores)
+    t0 = time.perf_counter()
+    oracle.lifelike(field, oracle.moore_deltas(2), oracle.moore_rev(2), 8, 12,
+                    steps, colors=False, threads=cores)
+    elapsed = time.perf_counter() - t0
+    return {
+        "value": SIZE[0] * SIZE[1] * steps / elapsed,
+        "unit": "cell-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": ("16384x16384 Life B3/S23, %d steps after 1 warm-up step, "
+                   "reference plain emit/absorb structure with 8 buffer "
+                   "planes (no colours), C + OpenMP" % steps),
+    }
+
+
+def plain_engine(steps=20):
+    """Cell-updates/s of the reference-structured GPU path on this field."""
+    from examples.game_of_life import GameOfLife
+    from examples.workloads import Life16kExperiment
+    from xentica_amd.runtime import hip
+    model = GameOfLife(Life16kExperiment)
+    model.step()
+    hip.device().synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        model.step()
+    hip.device().synchronize()
+    elapsed = time.perf_counter() - t0
+    return {"value": model.cells_num * steps / elapsed,
+            "unit": "cell-updates/s", "steps": steps,
+            "structure": "generated emit + absorb, 8 buffer planes + colours "
+                         "in HBM, one launch pair per step"}
+
+
+def main():
+    parser = argparse.ArgumentParser()
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=2000)
+    parser.add_argument("--warmup", type=int, default=200)
+    parser.add_argument("--depth", type=int, default=0,
+                        help="temporal depth (0 = autotune)")
+    parser.add_argument("--no-cpu", action="store_true")
+    parser.add_argument("--no-plain", action="store_true")
+    args = parser.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        return 2
+
+    from xentica_amd.runtime import hip
+    from xentica_amd.runtime.lifebits import LifeBitsField
+
+    device = hip.device()
+    log("device: %s" % device.name)
+    cells = seed_cells()
+    stream = hip.Stream()
+    field = LifeBitsField(SIZE[0], SIZE[1], born=8, sustain=12, stream=stream)
+    field.set_cells(cells)
+    depth = args.depth or autotune(field, stream)
+    field.depth = depth
+    field.set_cells(cells)
+    log("depth %d; warmup %d steps" % (depth, args.warmup))
+    if args.warmup:
+        timed(field, stream, args.warmup)
+    ms = timed(field, stream, args.steps)
+    n = SIZE[0] * SIZE[1]
+    value = n * args.steps / (ms / 1e3)
+    launches = -(-args.steps // depth)
+    launch_ms = ms / launches
+    bytes_per_launch = n * depth * BYTES_PER_CELL_UPDATE
+    achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9
+    log("%d steps in %.3f ms: %.4g cell-updates/s" % (args.steps, ms, value))
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u1",
+        "data": "synthetic",
+        "config": {
+            "workload": "life_b3s23_torus_16384x16384_bitpacked",
+            "size": list(SIZE), "rule": "B3/S23", "seed_word": WORD,
+            "density": 0.5, "temporal_depth": depth,
+            "layout": "64 cells per uint64 (32 per uint32 lane word)",
+            "device": device.name,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_per_launch(),
+        },
+    }
+    if not args.no_plain:
+        log("plain engine ...")
+        result["plain_engine"] = plain_engine()
+        result["speedup_vs_plain"] = value / result["plain_engine"]["value"]
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(cells)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
