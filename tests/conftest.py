@@ -20,3 +20,19 @@ def _cache_dir(tmp_path_factory, monkeypatch):
     cache = tmp_path_factory.getbasetemp() / "xentica_cache"
     cache.mkdir(exist_ok=True)
     monkeypatch.setenv("XENTICA_CACHE_DIR", str(cache))
+
+
+@pytest.fixture(autouse=True)
+def _gpu_progress(request):
+    """Name each GPU test on stderr before it touches the device."""
+    import time
+    if request.node.get_closest_marker("gpu") is None:
+        yield
+        return
+    start = time.time()
+    sys.__stderr__.write("[gpu-test start] %s\n" % request.node.nodeid)
+    sys.__stderr__.flush()
+    yield
+    sys.__stderr__.write("[gpu-test end] %s %.1fs\n" % (
+        request.node.nodeid, time.time() - start))
+    sys.__stderr__.flush()

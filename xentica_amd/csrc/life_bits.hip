@@ -87,12 +87,14 @@ struct Rule<false> {
         const uint32_t k1 = x & k0;
         const uint32_t c2 = y ^ k1;
         const uint32_t c3 = y & k1;
+        // leaf[v]: next state for count v; a live cell is kept when v is
+        // a birth OR a survival count (next = born | sustained & state)
         uint32_t leaf[9];
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
             const uint32_t b = ((born >> v) & 1u) ? 0xFFFFFFFFu : 0u;
             const uint32_t s = ((sustain >> v) & 1u) ? 0xFFFFFFFFu : 0u;
-            leaf[v] = bitop3<kMux>(m, s, b);
+            leaf[v] = bitop3<kMux>(m, s | b, b);
         }
         const uint32_t l0 = bitop3<kMux>(c0, leaf[1], leaf[0]);
         const uint32_t l1 = bitop3<kMux>(c0, leaf[3], leaf[2]);
