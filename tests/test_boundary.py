@@ -50,3 +50,18 @@ def test_compile_error_is_reported():
     with pytest.raises(XenticaException, match="hiprtc"):
         hip.compile_source('extern "C" __global__ void k() { nonsense; }',
                            "bad.hip")
+
+
+def test_device_first_call_does_not_deadlock():
+    """``hip.device()`` before anything else loads the library under its
+    lock; it must return or raise, never block (no GPU here: it raises)."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from xentica_amd.runtime import hip\n"
+            "from xentica_amd.core.exceptions import XenticaException\n"
+            "try:\n    hip.device()\nexcept XenticaException:\n    pass\n"
+            "print('done')\n") % ROOT
+    proc = subprocess.run([sys.executable, "-c", code], capture_output=True,
+                          text=True, timeout=60)
+    assert "done" in proc.stdout, proc.stderr

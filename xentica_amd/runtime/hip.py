@@ -36,7 +36,7 @@ COMPILE_OPTIONS = (
     "-w",
 )
 
-_LOCK = threading.Lock()
+_LOCK = threading.RLock()
 _LIB = None
 
 _vp = ctypes.c_void_p
