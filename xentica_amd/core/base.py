@@ -331,10 +331,11 @@ class Translator:
 class GPUKernels:
     """The compiled plain kernels of a model (``base.py:527-541``)."""
 
-    def __init__(self, source):
+    def __init__(self, source, fp_contract="off"):
         """Compile ``source`` for gfx950 and look the kernels up."""
         from xentica_amd.runtime import hip
-        module = hip.Module.from_source(source, "bsca_plain.hip")
+        module = hip.Module(hip.compile_source(
+            source, "bsca_plain.hip", hip.compile_options(fp_contract)))
         self.module = module
         self.emit = module.get_function("emit")
         self.absorb = module.get_function("absorb")
@@ -358,9 +359,10 @@ class GPUArrays:
 class GPU:
     """Kernels and arrays of a model; kernels compile on first use."""
 
-    def __init__(self, source, init_cells, init_colors):
+    def __init__(self, source, init_cells, init_colors, fp_contract="off"):
         """Keep the source; allocate nothing on the device yet."""
         self._source = source
+        self._fp_contract = fp_contract
         self._kernels = None
         self.arrays = GPUArrays(init_cells, init_colors)
 
@@ -368,7 +370,7 @@ class GPU:
     def kernels(self):
         """The compiled plain kernels."""
         if self._kernels is None:
-            self._kernels = GPUKernels(self._source)
+            self._kernels = GPUKernels(self._source, self._fp_contract)
         return self._kernels
 
 
@@ -393,6 +395,11 @@ class CellularAutomaton(Translator, metaclass=BSCA):
     #: Step engine: ``"plain"`` (reference structure) or any name known to
     #: ``xentica_amd.runtime.engines``; an experiment may override it.
     engine = "plain"
+    #: Floating-point contraction of the generated code: ``"off"`` keeps
+    #: every ``a * b + c`` as two roundings, ``"fast"`` lets the compiler
+    #: fuse them (the default of the reference's nvcc); an experiment may
+    #: override it.
+    fp_contract = "off"
 
     def __init__(self, experiment_class):
         """Translate the model, seed the field and prepare the GPU state."""
@@ -425,7 +432,8 @@ class CellularAutomaton(Translator, metaclass=BSCA):
         cells_total = self.cells_num * (len(self.buffers) + 1) + 1
         init_cells = np.zeros((cells_total, ), dtype=self.main.dtype)
         experiment_class.seed.generate(init_cells, self)
-        self.gpu = GPU(self.cuda_source, init_cells, init_colors)
+        self.gpu = GPU(self.cuda_source, init_cells, init_colors,
+                       self.fp_contract)
         self.bridge = MoireBridge
         self.renderer.setup_actions(self.bridge)
         self._lock = threading.Lock()

@@ -22,7 +22,7 @@ from xentica_amd.core.exceptions import XenticaException
 __all__ = [
     'library', 'check', 'Device', 'device', 'Module', 'Function',
     'Buffer', 'Stream', 'Event', 'Graph', 'compile_source',
-    'COMPILE_OPTIONS', 'library_path',
+    'COMPILE_OPTIONS', 'compile_options', 'library_path',
 ]
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -35,6 +35,15 @@ COMPILE_OPTIONS = (
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-w",
 )
+
+
+def compile_options(fp_contract="off"):
+    """hiprtc options with the given FP contraction mode (off/on/fast)."""
+    if fp_contract not in ("off", "on", "fast"):
+        raise XenticaException("fp_contract must be off, on or fast")
+    return tuple("-ffp-contract=%s" % fp_contract
+                 if opt.startswith("-ffp-contract=") else opt
+                 for opt in COMPILE_OPTIONS)
 
 _LOCK = threading.RLock()
 _LIB = None
