@@ -5,9 +5,12 @@ Benchmark and parity workloads, sized as the project's measurement plan.
   random state from seed 42, used for per-step hash parity.
 * :class:`Life16kExperiment` -- the same at 16384x16384 (the headline
   throughput workload, run by the bit-packed engine).
+* :class:`LifeLikeRule` and :func:`batch_experiments` -- the batch of 256
+  Life-like automata of uneven sizes, each with its own rule and density.
 """
 from xentica_amd import core
 from xentica_amd import seeds
+from xentica_amd.core import color_effects
 
 
 class Life512Experiment(core.Experiment):
@@ -25,3 +28,77 @@ class Life16kExperiment(Life512Experiment):
     """16384x16384, ``RandInt(0, 1)`` everywhere, word 42."""
 
     size = (16384, 16384)
+
+
+class LifeLikeRule(core.CellularAutomaton):
+    """
+    Life-like automaton whose rule is an interactive parameter.
+
+    Birth counts in bits 0-8 and survival counts in bits 9-17 of ``rule``
+    (the encoding of ``LifeLike.golly2int``); ``next = born | sustained &
+    state`` as in the reference's ``LifelikeCA`` (without its rule switch).
+    """
+
+    state = core.IntegerProperty(max_val=1)
+    rule = core.Parameter(default=6152, interactive=True)
+
+    class Topology:
+        """2-D Moore torus."""
+
+        dimensions = 2
+        lattice = core.OrthogonalLattice()
+        neighborhood = core.MooreNeighborhood()
+        border = core.TorusBorder()
+
+    def emit(self):
+        """Every buffer carries the state."""
+        for buf in self.buffers:
+            buf.state = self.main.state
+
+    def absorb(self):
+        """Apply the rule to the live-neighbour count."""
+        count = core.IntegerVariable()
+        for i in range(len(self.buffers)):
+            count += self.neighbors[i].buffer.state
+        is_born = (self.rule >> count) & 1
+        is_sustain = (self.rule >> 9 >> count) & 1
+        self.main.state = is_born | is_sustain & self.main.state
+
+    @color_effects.MovingAverage
+    def color(self):
+        """Live cells white."""
+        val = self.main.state * 255
+        return (val, val, val)
+
+
+def batch_experiments(count=256, seed=42, side_min=64, side_max=2048):
+    """
+    Experiments of the batched workload: ``count`` square tori.
+
+    Sides are ``round(exp(U(ln side_min, ln side_max)))``, birth and
+    survival masks uniform 9-bit, density uniform in [0.1, 0.6] quantised
+    to d/1024 (seeded as ``(RandInt(0, 1023) + d) >> 10``), word
+    ``seed + index``; drawn from ``numpy.random.default_rng(seed)``.
+    """
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sides = np.round(np.exp(rng.uniform(np.log(side_min), np.log(side_max),
+                                        count))).astype(int)
+    born = rng.integers(0, 512, count)
+    survive = rng.integers(0, 512, count)
+    density = rng.uniform(0.1, 0.6, count)
+    experiments = []
+    for idx in range(count):
+        level = int(round(density[idx] * 1024))
+        attrs = {
+            "word": seed + idx,
+            "size": (int(sides[idx]), int(sides[idx])),
+            "zoom": 1,
+            "pos": [0, 0],
+            "rule": int(born[idx]) | (int(survive[idx]) << 9),
+            "seed": seeds.patterns.PrimordialSoup(vals={
+                "state": (seeds.random.RandInt(0, 1023) + level) >> 10}),
+        }
+        experiments.append(type("BatchExperiment%d" % idx,
+                                (core.Experiment,), attrs))
+    return experiments

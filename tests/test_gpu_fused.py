@@ -1,0 +1,139 @@
+"""
+The fused engine (emit+absorb in one generated kernel, buffers in LDS,
+several steps per launch) against the reference's golden checksums, the
+plain engine and the oracle; and batches of uneven automata with their own
+rules against single runs.
+"""
+import binascii
+
+import numpy as np
+import pytest
+
+from examples.game_of_life import (
+    GameOfLife, GameOfLifeStatic, GameOfLifeColor, GOLExperiment,
+    GOLExperimentColor, LifelikeCA, DiamoebaExperiment)
+from examples.shifting_sands import ShiftingSands, ShiftingSandsExperiment
+from examples.noisetv import NoiseTV, NoiseTVExperiment
+from examples.evolife import (
+    EvoLife, CrossbreedingExperiment, CrossbreedingExperiment2)
+from examples.workloads import (
+    Life512Experiment, LifeLikeRule, batch_experiments)
+from xentica_amd.runtime.fused import Batch, FusedEngine
+
+pytestmark = pytest.mark.gpu
+
+
+def fused(exp):
+    """The experiment with the fused engine selected."""
+    return type(exp.__name__ + "Fused", (exp,), {"engine": "fused"})
+
+
+def run(model, steps):
+    for _ in range(steps):
+        model.step()
+    return model
+
+
+def crc(model):
+    return binascii.crc32(model.gpu.arrays.cells.get()[:model.cells_num])
+
+
+@pytest.mark.parametrize("cls,exp,steps,want", [
+    (GameOfLife, GOLExperiment, 1000, 2981695958),
+    (GameOfLifeStatic, GOLExperiment, 1000, 1098273940),
+    (GameOfLifeColor, GOLExperimentColor, 1000, 532957133),
+    (LifelikeCA, DiamoebaExperiment, 1000, 4016906756),
+    (ShiftingSands, ShiftingSandsExperiment, 1000, 1639048270),
+    (NoiseTV, NoiseTVExperiment, 100, 2773894957),
+])
+def test_golden(cls, exp, steps, want):
+    model = cls(fused(exp))
+    assert isinstance(model._engine, FusedEngine)
+    assert crc(run(model, steps)) == want
+
+
+def test_evolife_golden():
+    """``test_base.py:197-216`` on the fused engine (uint64 words)."""
+    model = run(EvoLife(fused(CrossbreedingExperiment),
+                        legacy_coloring=False), 100)
+    assert crc(model) == 2061779088
+    assert binascii.crc32(model.gpu.arrays.colors.get()) == 68940598
+    model = run(EvoLife(fused(CrossbreedingExperiment2)), 100)
+    assert crc(model) == 3381214984
+    assert binascii.crc32(model.gpu.arrays.colors.get()) == 2253830262
+
+
+def test_render_after_fused_steps():
+    class RenderExperiment(fused(GOLExperiment)):
+        zoom = 1
+    model = GameOfLife(RenderExperiment)
+    model.set_viewport(RenderExperiment.size)
+    run(model, 1000)
+    assert binascii.crc32(model.render() / 255) == 1955702083
+
+
+def test_colors_match_plain_every_step():
+    plain = GameOfLifeColor(GOLExperimentColor)
+    fast = GameOfLifeColor(fused(GOLExperimentColor))
+    for step in range(40):
+        plain.step()
+        fast.step()
+        assert crc(fast) == crc(plain), step
+        assert np.array_equal(fast.gpu.arrays.colors.get(),
+                              plain.gpu.arrays.colors.get()), step
+
+
+def test_life512_deferred_vs_oracle():
+    """1000 deferred steps, checked against the oracle every 100."""
+    from oracle import oracle as O
+    model = GameOfLife(fused(Life512Experiment))
+    start = model.gpu.arrays.cells.get()[:model.cells_num]
+    field = O.Field((512, 512), 8, np.uint8, start)
+    for block in range(10):
+        run(model, 100)
+        O.lifelike(field, O.moore_deltas(2), O.moore_rev(2), 8, 12, 100,
+                   colors=False)
+        assert np.array_equal(model.gpu.arrays.cells.get()[:model.cells_num],
+                              field.cells), block
+
+
+def test_save_load_fused(tmp_path):
+    path = str(tmp_path / "fused.ca")
+    ca1 = run(GameOfLife(fused(GOLExperiment)), 500)
+    ca1.save(path)
+    ca2 = GameOfLife(fused(GOLExperiment))
+    ca2.load(path)
+    assert crc(run(ca2, 500)) == 2981695958
+
+
+def test_batch_matches_single_runs():
+    """Uneven sizes and per-instance rules: batch == each model alone."""
+    exps = batch_experiments(count=12, seed=7, side_min=20, side_max=300)
+    batch_models = [LifeLikeRule(exp) for exp in exps]
+    batch = Batch(batch_models)
+    batch.run(37)
+    batch.synchronize()
+    batch.scatter()
+    for exp, model in zip(exps, batch_models):
+        single = LifeLikeRule(exp)
+        single.rule = exp.rule
+        run(single, 37)
+        assert crc(model) == crc(single), exp.__name__
+        assert np.array_equal(model.gpu.arrays.colors.get(),
+                              single.gpu.arrays.colors.get())
+
+
+def test_batch_graph_replays():
+    """The captured graph replayed several times equals eager launches."""
+    exps = batch_experiments(count=6, seed=3, side_min=40, side_max=200)
+    eager = Batch([LifeLikeRule(exp) for exp in exps])
+    graphed = Batch([LifeLikeRule(exp) for exp in exps])
+    for _ in range(3):
+        eager.run(20, graph=False)
+        graphed.run(20, graph=True)
+        eager.synchronize()
+        graphed.synchronize()
+        eager.scatter()
+        graphed.scatter()
+        for a, b in zip(eager.models, graphed.models):
+            assert crc(a) == crc(b)

@@ -12,7 +12,7 @@ main state and plane ``p`` buffer slot ``p`` (``xentica/core/base.py:659-661``,
 arithmetic with wrap or static-border handling (``properties.py:431-494``).
 """
 
-__all__ = ['PlainAddressing', ]
+__all__ = ['PlainAddressing', 'TileAddressing', ]
 
 
 class PlainAddressing:
@@ -32,6 +32,10 @@ class PlainAddressing:
         if container.buf_num > 0:
             return "fld[i + n * %d]" % container.buf_num
         return "fld[i]"
+
+    def store_ref(self, container):
+        """C lvalue the container's packed word is stored to."""
+        return self.cell_ref(container)
 
     def neighbor_declaration(self, container):
         """Coordinates, border handling and the load of a neighbour word."""
@@ -62,3 +66,86 @@ class PlainAddressing:
             var, border.off_board_state("_nx"),
             var, self.cell_ref(container))
         return code
+
+
+class TileAddressing:
+    """
+    LDS tile layout of the fused kernels.
+
+    A workgroup holds a region of the field (output tile plus halo) in
+    LDS: two main-state planes ``_mp[2][_RA]`` (read plane ``_cur``,
+    written plane ``_cur ^ 1``) and ``_K`` buffer planes ``_tb[_K * _RA]``;
+    ``_li`` is the cell's index in the region and ``_RW``/``_RH`` its
+    width and height.  Neighbours are fixed offsets in the region: the
+    kernel resolved torus wrapping or the static border when it loaded the
+    region, so no coordinate code is generated.
+
+    While translating, it records what the model does in each phase
+    (``phase`` is set by the translator to ``"emit"`` or ``"absorb"``):
+    which buffer slots emit stores, whether emit stores the main state or
+    reads neighbours' main state, and whether absorb stores buffers.
+    """
+
+    name = "tile"
+
+    def __init__(self, model):
+        self._model = model
+        self.phase = None
+        self.emit_slots = set()
+        self.emit_writes_main = False
+        self.emit_reads_neighbor_main = False
+        self.emit_reads_neighbor_buffer = False
+        self.absorb_writes_buffer = False
+
+    def _offset(self, nbr_num):
+        delta = self._model.topology.neighborhood.neighbor_deltas[nbr_num]
+        terms = []
+        stride = ""
+        for axis, step in enumerate(delta):
+            if axis == 0:
+                stride = ""
+            elif axis == 1:
+                stride = "_RW"
+            else:
+                stride = "_RW * _RH"
+            if step:
+                terms.append("(%d)%s" % (step, (" * " + stride) if stride else ""))
+        return " + ".join(terms) if terms else "0"
+
+    def _plane(self, buf_num, write=False):
+        if buf_num == 0:
+            return "_mp[_cur ^ 1]" if write else "_mp[_cur]"
+        return "_tb + %d * _RA" % (buf_num - 1)
+
+    def cell_ref(self, container):
+        """Load expression of a container word."""
+        if container.nbr_num >= 0:
+            if self.phase == "emit":
+                if container.buf_num == 0:
+                    self.emit_reads_neighbor_main = True
+                else:
+                    self.emit_reads_neighbor_buffer = True
+            plane = self._plane(container.buf_num)
+            return "(%s)[_li + %s]" % (plane, self._offset(container.nbr_num))
+        return "(%s)[_li]" % self._plane(container.buf_num)
+
+    def store_ref(self, container):
+        """Store target of a container word."""
+        if container.nbr_num >= 0:
+            raise NotImplementedError("stores to neighbour words")
+        if container.buf_num == 0:
+            if self.phase == "emit":
+                self.emit_writes_main = True
+            return "(%s)[_li]" % self._plane(0, write=True)
+        if self.phase == "emit":
+            self.emit_slots.add(container.buf_num)
+        else:
+            self.absorb_writes_buffer = True
+        return "(%s)[_li]" % self._plane(container.buf_num)
+
+    def neighbor_declaration(self, container):
+        """Load of a neighbour word from the region."""
+        if container.init_val is None:
+            return "%s %s;\n" % (container.ctype, container.var_name)
+        return "%s %s = %s;\n" % (container.ctype, container.var_name,
+                                   self.cell_ref(container))

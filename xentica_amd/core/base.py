@@ -29,7 +29,7 @@ import numpy as np
 from xentica_amd.bridge import MoireBridge
 from xentica_amd.seeds.random import LocalRandom
 from xentica_amd.core import _context
-from xentica_amd.core.addressing import PlainAddressing
+from xentica_amd.core.addressing import PlainAddressing, TileAddressing
 from xentica_amd.core.properties import Property, ContainerProperty
 from xentica_amd.core.renderers import RendererPlain
 from xentica_amd.core.exceptions import XenticaException
@@ -313,6 +313,27 @@ class Translator:
         self.kernel_bodies['render'] = body
         return _elementwise_kernel("render", self.renderer.args, body)
 
+    def translate_tile(self):
+        """
+        Translate emit and absorb+color for the fused LDS-tile kernels.
+
+        Returns ``(emit_body, absorb_body, addressing)``; the addressing
+        object records the access pattern the kernel must honour.  The
+        plain source is left untouched.
+        """
+        plain = self.addressing
+        tile = TileAddressing(self)
+        self.addressing = tile
+        try:
+            tile.phase = "emit"
+            emit_body = self._translate_code(self.emit)
+            tile.phase = "absorb"
+            absorb_body = self._translate_code(self.absorb, self.color)
+            tile.phase = None
+        finally:
+            self.addressing = plain
+        return emit_body, absorb_body, tile
+
     def index_to_coord(self, i):
         """Coordinates of flat index ``i`` (or an index array)."""
         return self.topology.lattice.index_to_coord(i, self)
@@ -449,6 +470,10 @@ class CellularAutomaton(Translator, metaclass=BSCA):
     def absorb_params(self):
         """Interactive parameters passed to ``absorb``, in argument order."""
         return list(self._absorb_params)
+
+    def flush(self):
+        """Finish steps the engine has deferred (reads do this implicitly)."""
+        self._engine.flush()
 
     def apply_speed(self, dval):
         """Change the simulation speed (never below 1)."""

@@ -83,5 +83,13 @@ class Parameter(BscaDetectorMixin):
         return self._value
 
     def __set__(self, obj, value):
-        """Change the value (read by the next step if interactive)."""
+        """
+        Change the value (read by the next step if interactive).
+
+        The value is class-wide, as in the reference; a model instance also
+        remembers its own value so that batched runs can give every
+        instance its own parameters.
+        """
         self._value = value
+        if obj is not None and hasattr(obj, "__dict__"):
+            obj.__dict__.setdefault("_param_values", {})[self._name] = value

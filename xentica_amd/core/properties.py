@@ -356,7 +356,8 @@ class ContainerProperty(Property):
             vals.append(val)
             shift += prop.bit_width
         code = "%s = %s;\n" % (self.var_name, " + ".join(vals))
-        code += "%s = %s;\n" % (self._mem_cell(), self.var_name)
+        code += "%s = %s;\n" % (
+            self.bsca.addressing.store_ref(self), self.var_name)
         self.bsca.append_code(code)
 
 

@@ -5,6 +5,10 @@ Step engines: how a model's step is executed on the GPU.
   one ``emit`` launch writing every buffer plane to HBM, then one
   ``absorb`` launch reading them, per step.  Works for every model the DSL
   can express and is the parity bridge for the faster engines.
+* ``fused`` -- emit and absorb fused in one generated kernel with the
+  buffers in LDS and several steps per launch (``runtime/fused.py``);
+  steps are deferred until a launch is full or the state is read.
+* ``auto`` -- ``fused`` when the model qualifies, else ``plain``.
 
 Engines share one interface: ``step()`` (advance one step, possibly
 deferred), ``flush()`` (finish deferred steps), ``load_state(cells,
@@ -71,8 +75,27 @@ class PlainEngine:
         arrays.colors.set(colors)
 
 
+def _fused(model):
+    from xentica_amd.runtime.fused import FusedEngine
+    return FusedEngine(model)
+
+
+def _auto(model):
+    """The fused engine when the model qualifies, else the plain one."""
+    from xentica_amd.runtime.fused import FusedEngine, eligible
+    ok, _ = eligible(model)
+    if ok:
+        try:
+            return FusedEngine(model)
+        except XenticaException:
+            pass
+    return PlainEngine(model)
+
+
 ENGINES = {
     'plain': PlainEngine,
+    'fused': _fused,
+    'auto': _auto,
 }
 
 

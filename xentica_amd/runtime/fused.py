@@ -1,0 +1,555 @@
+"""
+Fused emit+absorb kernels: buffers in LDS, several steps per launch.
+
+For a model on a 2-D orthogonal lattice (Moore, von Neumann or any
+orthogonal neighbourhood with unit offsets) with a torus or static
+border, the DSL methods are translated a second time with the tile
+addressing scheme (``core/addressing.py:TileAddressing``) and wrapped in
+one generated kernel:
+
+* a workgroup owns a ``TX x TY`` output tile and loads the region around
+  it -- halo ``M = h * T`` where ``T`` is the number of steps per launch
+  and ``h`` is 1, or 2 when ``emit`` reads neighbours' main state -- into
+  LDS, resolving the torus wrap or the static border while loading;
+* each step runs ``emit`` over the cells whose inputs are still valid
+  (``distance to the region edge >= s*h + h - 1``), writing the buffer
+  slots to LDS planes, then ``absorb``+``color`` over the cells at
+  distance ``>= (s+1)*h`` into the other main-state plane, so buffers
+  never reach HBM and the main state makes one HBM round trip per ``T``
+  steps (the reference writes all ``k`` buffer planes to HBM every step,
+  ``xentica/core/base.py:254-283, 461-487``);
+* the colours of the output cells live in LDS for the whole launch, so
+  the per-step colour effect (``color_effects.py:106-130``) is exact;
+* the kernel is persistent over a list of (instance, tile) work items
+  described by a per-instance table (offsets, sizes, parameter values),
+  so one launch advances any number of independent automata of uneven
+  sizes and different parameters; a multi-step run is captured as a HIP
+  graph.
+
+Step semantics are those of the reference's plain structure: the emit of
+a step sees the main state before it, absorb sees every buffer written by
+that step's emit, slots emit never writes stay zero (as in the
+reference's zero-initialised buffer planes).
+"""
+import numpy as np
+
+from xentica_amd.core.exceptions import XenticaException
+from xentica_amd.core.topology.border import TorusBorder, StaticBorder
+from xentica_amd.core.topology.lattice import OrthogonalLattice
+from xentica_amd.core.topology.neighborhood import OrthogonalNeighborhood
+from xentica_amd.runtime import hip
+
+__all__ = ['FusedProgram', 'FusedEngine', 'Batch', 'eligible']
+
+
+class _DoesNotFit(XenticaException):
+    """The requested tile and depth exceed the LDS budget."""
+
+#: LDS one workgroup may use (two workgroups per CU fit in 160 KiB).
+LDS_BUDGET = 64 * 1024
+BLOCK = 256
+
+_PARAM_FIELD = {'int': ('int', np.int32), 'float': ('float', np.float32),
+                'bool': ('int', np.int32)}
+
+
+def eligible(model):
+    """``(True, '')`` if the fused kernels can run ``model``, else a reason."""
+    topology = model.topology
+    if topology.dimensions != 2:
+        return False, "fused kernels are 2-D"
+    if not isinstance(topology.lattice, OrthogonalLattice):
+        return False, "lattice is not orthogonal"
+    hood = topology.neighborhood
+    if not isinstance(hood, OrthogonalNeighborhood):
+        return False, "neighbourhood is not orthogonal"
+    if any(abs(step) > 1 for delta in hood.neighbor_deltas for step in delta):
+        return False, "neighbour offsets beyond 1"
+    if type(topology.border) not in (TorusBorder, StaticBorder):
+        return False, "border is neither TorusBorder nor StaticBorder"
+    for param in model._emit_params + model._absorb_params:
+        if param.ctype not in _PARAM_FIELD:
+            return False, "parameter %s has C type %s" % (param.name,
+                                                          param.ctype)
+    return True, ""
+
+
+class FusedProgram:
+    """The generated fused kernel of one model class and tile shape."""
+
+    #: (tile, depth) candidates, best first; the first that fits LDS wins.
+    CANDIDATES = (((32, 32), 4), ((64, 16), 4), ((32, 16), 4), ((32, 32), 2),
+                  ((32, 16), 2), ((16, 16), 2), ((16, 16), 1), ((8, 8), 1))
+
+    def __init__(self, model, tile=None, depth=None):
+        if tile is None or depth is None:
+            for cand_tile, cand_depth in self.CANDIDATES:
+                try:
+                    self.__init__(model, cand_tile, cand_depth)
+                    return
+                except _DoesNotFit:
+                    continue
+            raise XenticaException("fused engine: no tile fits in LDS")
+        ok, why = eligible(model)
+        if not ok:
+            raise XenticaException("fused engine: " + why)
+        emit_body, absorb_body, access = model.translate_tile()
+        if access.emit_writes_main:
+            raise XenticaException(
+                "fused engine: emit() stores the main state (the reference "
+                "races on it); use the plain engine")
+        if access.emit_reads_neighbor_buffer or access.absorb_writes_buffer:
+            raise XenticaException(
+                "fused engine: emit reads neighbour buffers or absorb "
+                "writes buffers")
+        self.halo_per_step = 2 if access.emit_reads_neighbor_main else 1
+        self.depth = int(depth)
+        self.tile = (int(tile[0]), int(tile[1]))
+        self.k = len(model.buffers)
+        self.word = model.main.dtype
+        self.ctype = model.main.ctype
+        self.params = list({p.name: p for p in model._emit_params
+                            + model._absorb_params}.values())
+        self.emit_slots = sorted(access.emit_slots)
+        self.static = isinstance(model.topology.border, StaticBorder)
+        self.border_value = (model.topology.border.value
+                             if self.static else 0)
+        margin = self.halo_per_step * self.depth
+        self.region = (self.tile[0] + 2 * margin, self.tile[1] + 2 * margin)
+        area = self.region[0] * self.region[1]
+        itemsize = np.dtype(self.word).itemsize
+        self.lds_bytes = ((2 + self.k) * area * itemsize
+                          + 12 * (self.tile[0] * self.tile[1] + BLOCK))
+        if self.lds_bytes > LDS_BUDGET:
+            raise _DoesNotFit(
+                "fused engine: tile %s at depth %d needs %d bytes of LDS" % (
+                    self.tile, self.depth, self.lds_bytes))
+        self.inst_dtype = self._instance_dtype()
+        defines = "".join(
+            const.get_define_code() for name, const in model.constants.items()
+            if not name.startswith(model.topology.lattice.width_prefix))
+        self.source = self._render(defines, emit_body, absorb_body, margin)
+        self._module = None
+
+    def _instance_dtype(self):
+        fields = [('cells', np.int64), ('cols', np.int64), ('w0', np.int32),
+                  ('w1', np.int32), ('tiles_x', np.int32), ('tile0', np.int32)]
+        for param in self.params:
+            fields.append(('p_' + param.name, _PARAM_FIELD[param.ctype][1]))
+        size = sum(np.dtype(t).itemsize for _, t in fields)
+        if size % 8:
+            fields.append(('pad', np.int32))
+        return np.dtype(fields, align=True)
+
+    def _render(self, defines, emit_body, absorb_body, margin):
+        tx, ty = self.tile
+        params_struct = "".join(
+            "%s p_%s;\n" % (_PARAM_FIELD[p.ctype][0], p.name)
+            for p in self.params)
+        if self.inst_dtype.names[-1] == 'pad':
+            params_struct += "int pad;\n"
+        param_locals = "".join(
+            "const %s %s = (%s) _in.p_%s;\n" % (p.ctype, p.name, p.ctype, p.name)
+            for p in self.params)
+        zero_slots = "".join(
+            "_tb[%d * _RA + _q] = 0;\n" % (slot - 1)
+            for slot in range(1, self.k + 1) if slot not in self.emit_slots)
+        # an off-board neighbour reads as the constant in every slot
+        static_emit = "".join(
+            "_tb[%d * _RA + _li] = (_word) _BVALUE;\n" % (slot - 1)
+            for slot in range(1, self.k + 1))
+        border = "1" if self.static else "0"
+        tb_decl = "__shared__ _word _tb[_K * _RA];\n" if self.k else \
+            "_word *_tb = 0;\n"
+        return _KERNEL_TEMPLATE.format(
+            defines=defines, tx=tx, ty=ty, margin=margin, k=self.k,
+            he=self.halo_per_step, ctype=self.ctype, static=border,
+            bvalue="%dULL" % (self.border_value & 0xFFFFFFFFFFFFFFFF),
+            params_struct=params_struct, param_locals=param_locals,
+            zero_slots=zero_slots, static_emit=static_emit, tb_decl=tb_decl,
+            emit_body=emit_body, absorb_body=absorb_body)
+
+    @property
+    def function(self):
+        """The compiled kernel (hiprtc, cached)."""
+        if self._module is None:
+            self._module = hip.Module.from_source(self.source, "bsca_fused.hip")
+        return self._module.get_function("fused_bsca")
+
+    def tiles_of(self, size):
+        """``(tiles_x, tiles_y)`` covering a field of ``size``."""
+        return (-(-size[0] // self.tile[0]), -(-size[1] // self.tile[1]))
+
+
+_KERNEL_TEMPLATE = r"""
+{defines}
+#define _TX {tx}
+#define _TY {ty}
+#define _M {margin}
+#define _RW (_TX + 2 * _M)
+#define _RH (_TY + 2 * _M)
+#define _RA (_RW * _RH)
+#define _K {k}
+#define _HE {he}
+#define _STATIC {static}
+#define _BVALUE {bvalue}
+typedef {ctype} _word;
+
+struct _Inst {{
+long long cells;
+long long cols;
+int w0;
+int w1;
+int tiles_x;
+int tile0;
+{params_struct}}};
+
+__device__ __forceinline__ int _wrap(int v, int n) {{
+    v %= n;
+    return v < 0 ? v + n : v;
+}}
+
+extern "C" __global__ __launch_bounds__(256)
+void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all,
+                int3 *__restrict__ _col_all, const _Inst *__restrict__ _insts,
+                const int *__restrict__ _tile_inst, int _n_tiles, int _steps) {{
+    __shared__ _word _mp[2][_RA];
+    {tb_decl}
+    __shared__ int3 _col[_TX * _TY + 256];
+    for (int _tile = blockIdx.x; _tile < _n_tiles; _tile += gridDim.x) {{
+        const _Inst _in = _insts[_tile_inst[_tile]];
+        const int _t = _tile - _in.tile0;
+        const int _ox = (_t % _in.tiles_x) * _TX;
+        const int _oy = (_t / _in.tiles_x) * _TY;
+        const _word *_src = _src_all + _in.cells;
+        _word *_dst = _dst_all + _in.cells;
+        int3 *_cg = _col_all + _in.cols;
+        {param_locals}
+        for (int _q = threadIdx.x; _q < _RA; _q += blockDim.x) {{
+            const int _gx = _ox - _M + _q % _RW, _gy = _oy - _M + _q / _RW;
+#if _STATIC
+            const bool _off = _gx < 0 || _gx >= _in.w0 || _gy < 0 || _gy >= _in.w1;
+            _mp[0][_q] = _off ? (_word) _BVALUE : _src[(long long) _gy * _in.w0 + _gx];
+#else
+            _mp[0][_q] = _src[(long long) _wrap(_gy, _in.w1) * _in.w0 + _wrap(_gx, _in.w0)];
+#endif
+            {zero_slots}
+        }}
+        for (int _q = threadIdx.x; _q < _TX * _TY; _q += blockDim.x) {{
+            const int _gx = _ox + _q % _TX, _gy = _oy + _q / _TX;
+            if (_gx < _in.w0 && _gy < _in.w1)
+                _col[_q] = _cg[(long long) _gy * _in.w0 + _gx];
+        }}
+        __syncthreads();
+        int _cur = 0;
+        for (int _s = 0; _s < _steps; ++_s) {{
+            const int _de = _s * _HE + (_HE - 1);
+            const int _da = (_s + 1) * _HE;
+            for (int _li = threadIdx.x; _li < _RA; _li += blockDim.x) {{
+                const int _lx = _li % _RW, _ly = _li / _RW;
+                const int _d = min(min(_lx, _RW - 1 - _lx), min(_ly, _RH - 1 - _ly));
+#if _STATIC
+                const int _gx = _ox - _M + _lx, _gy = _oy - _M + _ly;
+                if (_gx < 0 || _gx >= _in.w0 || _gy < 0 || _gy >= _in.w1) {{
+                    {static_emit}
+                    continue;
+                }}
+#endif
+                if (_d < _de) continue;
+                {{
+{emit_body}
+                }}
+            }}
+            __syncthreads();
+            for (int _li = threadIdx.x; _li < _RA; _li += blockDim.x) {{
+                const int _lx = _li % _RW, _ly = _li / _RW;
+                const int _d = min(min(_lx, _RW - 1 - _lx), min(_ly, _RH - 1 - _ly));
+#if _STATIC
+                const int _gx = _ox - _M + _lx, _gy = _oy - _M + _ly;
+                if (_gx < 0 || _gx >= _in.w0 || _gy < 0 || _gy >= _in.w1) {{
+                    _mp[_cur ^ 1][_li] = (_word) _BVALUE;
+                    continue;
+                }}
+#endif
+                if (_d < _da) continue;
+                _mp[_cur ^ 1][_li] = _mp[_cur][_li];
+                const int _oi = _d >= _M ? (_ly - _M) * _TX + (_lx - _M)
+                                         : _TX * _TY + (int) threadIdx.x;
+                {{
+                    int3 *col = &_col[_oi];
+                    const unsigned i = 0;
+{absorb_body}
+                }}
+            }}
+            __syncthreads();
+            _cur ^= 1;
+        }}
+        for (int _q = threadIdx.x; _q < _TX * _TY; _q += blockDim.x) {{
+            const int _qx = _q % _TX, _qy = _q / _TX;
+            const int _gx = _ox + _qx, _gy = _oy + _qy;
+            if (_gx < _in.w0 && _gy < _in.w1) {{
+                const long long _g = (long long) _gy * _in.w0 + _gx;
+                _dst[_g] = _mp[_cur][(_qy + _M) * _RW + _qx + _M];
+                _cg[_g] = _col[_q];
+            }}
+        }}
+        __syncthreads();
+    }}
+}}
+"""
+
+
+def _persistent_grid(n_tiles, lds_bytes):
+    """Workgroups of a persistent launch: all CUs, as many as LDS allows."""
+    per_cu = max(1, min(8, (160 * 1024) // max(1, lds_bytes)))
+    return max(1, min(n_tiles, 256 * per_cu))
+
+
+class _Instances:
+    """Device tables of a set of model instances run by one program."""
+
+    def __init__(self, program, models, per_instance=True):
+        self.program = program
+        self.models = list(models)
+        self.per_instance = per_instance
+        itemsize = np.dtype(program.word).itemsize
+        offsets, col_offsets, tile_inst = [], [], []
+        cells = cols = 0
+        tile0 = 0
+        table = np.zeros(len(self.models), dtype=program.inst_dtype)
+        for idx, model in enumerate(self.models):
+            if len(model.size) != 2:
+                raise XenticaException("fused engine: 2-D models only")
+            tiles_x, tiles_y = program.tiles_of(model.size)
+            table[idx]['cells'] = cells
+            table[idx]['cols'] = cols
+            table[idx]['w0'], table[idx]['w1'] = model.size
+            table[idx]['tiles_x'] = tiles_x
+            table[idx]['tile0'] = tile0
+            offsets.append(cells)
+            col_offsets.append(cols)
+            tile_inst += [idx] * (tiles_x * tiles_y)
+            tile0 += tiles_x * tiles_y
+            cells += model.cells_num
+            cols += model.cells_num
+        self.table = table
+        self.offsets = offsets
+        self.col_offsets = col_offsets
+        self.total_cells = cells
+        self.n_tiles = tile0
+        self.itemsize = itemsize
+        self.table_buf = hip.Buffer(max(1, table.nbytes))
+        self.tiles_buf = hip.Buffer(4 * max(1, self.n_tiles))
+        self.tiles_buf.upload(np.array(tile_inst, dtype=np.int32))
+        self._uploaded = None
+
+    def param_values(self):
+        """Per-instance parameter values (instance overrides first)."""
+        values = []
+        for model in self.models:
+            own = getattr(model, "_param_values", {}) if self.per_instance \
+                else {}
+            values.append(tuple(own.get(p.name, p.value)
+                                for p in self.program.params))
+        return values
+
+    def refresh(self, values=None):
+        """Upload the table when parameter values changed."""
+        if values is None:
+            values = self.param_values()
+        if values != self._uploaded:
+            for idx, vals in enumerate(values):
+                for param, val in zip(self.program.params, vals):
+                    self.table[idx]['p_' + param.name] = val
+            self.table_buf.upload(self.table)
+            self._uploaded = values
+
+
+class FusedEngine:
+    """Single-model engine on the fused kernel, with deferred steps."""
+
+    name = "fused"
+
+    def __init__(self, model, tile=None, depth=None):
+        self.model = model
+        self.program = FusedProgram(model, tile=tile, depth=depth)
+        self._inst = None
+        self._pending = 0
+        self._pending_values = None
+        self._other = None
+        self._in_other = False
+        arrays = model.gpu.arrays
+        arrays.cells.before_read = self.flush
+        arrays.colors.before_read = self.flush
+
+    def _setup(self):
+        if self._inst is None:
+            # a single model reads the class-wide parameter values, as the
+            # reference's step() does (xentica/core/base.py:717, 722)
+            self._inst = _Instances(self.program, [self.model],
+                                    per_instance=False)
+            nbytes = self.model.cells_num * self._inst.itemsize
+            self._other = hip.Buffer(nbytes)
+
+    def launch_elementwise(self, kernel, args, count):
+        """Grid-stride launch (render, through the plain kernels)."""
+        from xentica_amd.runtime.engines import elementwise_grid, BLOCK as EB
+        kernel(*args, grid=(elementwise_grid(count), 1, 1), block=(EB, 1, 1))
+
+    def step(self):
+        """Queue one step with the current parameter values."""
+        self._setup()
+        values = self._inst.param_values()
+        if self._pending and values != self._pending_values:
+            self.flush()
+        self._pending_values = values
+        self._pending += 1
+        if self._pending >= self.program.depth:
+            self.flush()
+
+    def _run(self, steps, values):
+        self._inst.refresh(values)
+        arrays = self.model.gpu.arrays
+        plane0 = arrays.cells.device_ptr()
+        colors = arrays.colors.device_ptr()
+        grid = _persistent_grid(self._inst.n_tiles, self.program.lds_bytes)
+        src, dst = ((self._other.ptr, plane0) if self._in_other
+                    else (plane0, self._other.ptr))
+        self.program.function(
+            np.uint64(src), np.uint64(dst), np.uint64(colors),
+            np.uint64(self._inst.table_buf.ptr),
+            np.uint64(self._inst.tiles_buf.ptr),
+            np.int32(self._inst.n_tiles), np.int32(steps),
+            grid=(grid, 1, 1), block=(BLOCK, 1, 1))
+        self._in_other = not self._in_other
+
+    def flush(self):
+        """Run the queued steps; leave the state in the cells array."""
+        while self._pending:
+            steps = min(self._pending, self.program.depth)
+            self._run(steps, self._pending_values)
+            self._pending -= steps
+        if self._in_other:
+            nbytes = self.model.cells_num * self._inst.itemsize
+            hip.check(hip.library().xen_memcpy_d2d(
+                self.model.gpu.arrays.cells.device_ptr(), self._other.ptr,
+                nbytes, None))
+            self._in_other = False
+
+    def load_state(self, cells, colors):
+        """Replace the state (queued steps are applied first)."""
+        self.flush()
+        arrays = self.model.gpu.arrays
+        arrays.cells.set(cells)
+        arrays.colors.set(colors)
+
+
+class Batch:
+    """
+    Many independent models of one class stepped together.
+
+    Every model keeps its own size, seed and parameter values (set through
+    ``model.<param> = value`` or its experiment); one persistent launch
+    advances all of them ``depth`` steps.  ``run(steps)`` captures the
+    launch sequence in a HIP graph and replays it.
+
+    :param models: instances of one ``CellularAutomaton`` subclass
+    :param tile: output tile of a workgroup
+    :param depth: steps per launch
+    """
+
+    def __init__(self, models, tile=(32, 32), depth=4, stream=None):
+        models = list(models)
+        if not models:
+            raise XenticaException("empty batch")
+        cls = type(models[0])
+        if any(type(m) is not cls for m in models):
+            raise XenticaException("a batch holds instances of one class")
+        self.program = FusedProgram(models[0], tile=tile, depth=depth)
+        self.models = models
+        self.stream = stream or hip.Stream()
+        self.inst = _Instances(self.program, models)
+        total = self.inst.total_cells * self.inst.itemsize
+        self.state = [hip.Buffer(total), hip.Buffer(total)]
+        self.colors = hip.Buffer(12 * self.inst.total_cells)
+        self.cur = 0
+        self._graphs = {}
+        self.gather()
+
+    def gather(self):
+        """Copy every model's current state into the batch buffers."""
+        from xentica_amd.runtime.hip import check, library
+        lib = library()
+        for model, off, coff in zip(self.models, self.inst.offsets,
+                                    self.inst.col_offsets):
+            model._engine.flush()
+            arrays = model.gpu.arrays
+            check(lib.xen_memcpy_d2d(
+                self.state[self.cur].ptr + off * self.inst.itemsize,
+                arrays.cells.device_ptr(),
+                model.cells_num * self.inst.itemsize, self.stream.handle))
+            check(lib.xen_memcpy_d2d(
+                self.colors.ptr + coff * 12, arrays.colors.device_ptr(),
+                model.cells_num * 12, self.stream.handle))
+        self.stream.synchronize()
+
+    def scatter(self):
+        """Copy the batch state back into every model's arrays."""
+        from xentica_amd.runtime.hip import check, library
+        lib = library()
+        for model, off, coff in zip(self.models, self.inst.offsets,
+                                    self.inst.col_offsets):
+            arrays = model.gpu.arrays
+            check(lib.xen_memcpy_d2d(
+                arrays.cells.device_ptr(),
+                self.state[self.cur].ptr + off * self.inst.itemsize,
+                model.cells_num * self.inst.itemsize, self.stream.handle))
+            check(lib.xen_memcpy_d2d(
+                arrays.colors.device_ptr(), self.colors.ptr + coff * 12,
+                model.cells_num * 12, self.stream.handle))
+        self.stream.synchronize()
+
+    def _launch(self, steps):
+        grid = _persistent_grid(self.inst.n_tiles, self.program.lds_bytes)
+        self.program.function(
+            np.uint64(self.state[self.cur].ptr),
+            np.uint64(self.state[1 - self.cur].ptr),
+            np.uint64(self.colors.ptr), np.uint64(self.inst.table_buf.ptr),
+            np.uint64(self.inst.tiles_buf.ptr), np.int32(self.inst.n_tiles),
+            np.int32(steps), grid=(grid, 1, 1), block=(BLOCK, 1, 1),
+            stream=self.stream)
+        self.cur = 1 - self.cur
+
+    def launch_plan(self, steps):
+        """Steps per launch of a run of ``steps`` steps."""
+        depth = self.program.depth
+        return [depth] * (steps // depth) + ([steps % depth] if steps % depth
+                                              else [])
+
+    def run(self, steps, graph=True):
+        """Advance every instance ``steps`` steps (asynchronous)."""
+        self.inst.refresh()
+        self.program.function  # compile before capture
+        plan = self.launch_plan(steps)
+        if not graph:
+            for count in plan:
+                self._launch(count)
+            return
+        key = (tuple(plan), self.cur)
+        if key not in self._graphs:
+            start = self.cur
+            hip.Graph.begin(self.stream)
+            try:
+                for count in plan:
+                    self._launch(count)
+            finally:
+                graph_obj = hip.Graph.end(self.stream)
+            self._graphs[key] = (graph_obj, self.cur)
+            self.cur = start
+        graph_obj, end = self._graphs[key]
+        graph_obj.launch(self.stream)
+        self.cur = end
+
+    def synchronize(self):
+        """Wait for the batch's stream."""
+        self.stream.synchronize()
