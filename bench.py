@@ -115,7 +115,9 @@ def plain_engine(steps=20):
     from examples.game_of_life import GameOfLife
     from examples.workloads import Life16kExperiment
     from xentica_amd.runtime import hip
-    model = GameOfLife(Life16kExperiment)
+    class PlainLife16k(Life16kExperiment):
+        engine = "plain"
+    model = GameOfLife(PlainLife16k)
     model.step()
     hip.device().synchronize()
     t0 = time.perf_counter()
@@ -129,6 +131,58 @@ def plain_engine(steps=20):
                          "in HBM, one launch pair per step"}
 
 
+def batched(steps=1000, plain_steps=10):
+    """
+    The batch workload: 256 Life-like tori, sides 64-2048, own rules.
+
+    Batched: one persistent fused launch per 4 steps over every instance,
+    the 1000-step run captured in a HIP graph.  Baseline: the per-instance
+    plain launch loop (two launches per instance per step), timed over
+    ``plain_steps`` steps.
+    """
+    from examples.workloads import LifeLikeRule, batch_experiments
+    from xentica_amd.runtime import hip
+    from xentica_amd.runtime.fused import Batch
+    exps = batch_experiments()
+    models = [LifeLikeRule(exp) for exp in exps]
+    total = sum(model.cells_num for model in models)
+    batch = Batch(models)
+    batch.run(steps)                       # compile, capture, warm up
+    batch.synchronize()
+    start, stop = hip.Event(), hip.Event()
+    start.record(batch.stream)
+    batch.run(steps)
+    stop.record(batch.stream)
+    stop.synchronize()
+    ms = start.elapsed_ms(stop)
+    value = total * steps / (ms / 1e3)
+    plain_models = []
+    for exp in exps:
+        plain_exp = type(exp.__name__ + "Plain", (exp,), {"engine": "plain"})
+        plain_models.append(LifeLikeRule(plain_exp))
+    for model, exp in zip(plain_models, exps):
+        model.rule = exp.rule
+        model.step()
+    hip.device().synchronize()
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        for model, exp in zip(plain_models, exps):
+            model.rule = exp.rule
+            model.step()
+    hip.device().synchronize()
+    plain_value = total * plain_steps / (time.perf_counter() - t0)
+    return {
+        "workload": "256 Life-like tori, sides log-uniform 64-2048, random "
+                    "9-bit birth/survival masks, density 0.1-0.6, %d steps"
+                    % steps,
+        "cells": total, "value": value, "unit": "cell-updates/s",
+        "ms": ms, "tile": list(batch.program.tile),
+        "depth": batch.program.depth,
+        "plain_loop_value": plain_value, "plain_loop_steps": plain_steps,
+        "speedup_vs_plain": value / plain_value,
+    }
+
+
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -138,6 +192,7 @@ def main():
                         help="temporal depth (0 = autotune)")
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--no-plain", action="store_true")
+    parser.add_argument("--no-batch", action="store_true")
     args = parser.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -201,6 +256,9 @@ def main():
         log("plain engine ...")
         result["plain_engine"] = plain_engine()
         result["speedup_vs_plain"] = value / result["plain_engine"]["value"]
+    if not args.no_batch:
+        log("batched workload ...")
+        result["batched"] = batched()
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(cells)

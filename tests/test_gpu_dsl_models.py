@@ -18,6 +18,13 @@ from examples.gravicowboys import GraviCowboys, GraviCowboysExperiment
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(params=["plain", "auto"], autouse=True)
+def _engine(request, monkeypatch):
+    """Run every model on the plain engine and on the default one."""
+    from xentica_amd.core.base import CellularAutomaton
+    monkeypatch.setattr(CellularAutomaton, "engine", request.param)
+
+
 def run(model, steps):
     for _ in range(steps):
         model.step()

@@ -118,7 +118,9 @@ def test_life16k_vs_plain_engine():
     """500 steps at 16384x16384: bit-packed and plain engines agree."""
     from examples.game_of_life import GameOfLife
     from examples.workloads import Life16kExperiment
-    model = GameOfLife(Life16kExperiment)
+    class PlainLife16k(Life16kExperiment):
+        engine = "plain"
+    model = GameOfLife(PlainLife16k)
     cells = model.gpu.arrays.cells.get()[:model.cells_num]
     life = LifeBitsField(16384, 16384, 8, 12, depth=4)
     life.set_cells(cells)

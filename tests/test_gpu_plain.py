@@ -29,6 +29,13 @@ pytestmark = pytest.mark.gpu
 NUM_STEPS = 1000
 
 
+@pytest.fixture(autouse=True)
+def _plain_engine(monkeypatch):
+    """Every model in this file runs on the reference-structured engine."""
+    from xentica_amd.core.base import CellularAutomaton
+    monkeypatch.setattr(CellularAutomaton, "engine", "plain")
+
+
 def run(model, steps):
     for _ in range(steps):
         model.step()
@@ -41,7 +48,9 @@ def cells_crc(model):
 
 def test_single_ca():
     """``test_base.py:28-41``."""
+    from xentica_amd.runtime.engines import PlainEngine
     model = run(GameOfLife(GOLExperiment), NUM_STEPS)
+    assert isinstance(model._engine, PlainEngine)
     assert cells_crc(model) == 2981695958
 
 

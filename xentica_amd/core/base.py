@@ -413,9 +413,10 @@ class CellularAutomaton(Translator, metaclass=BSCA):
         parameter values.
     """
 
-    #: Step engine: ``"plain"`` (reference structure) or any name known to
-    #: ``xentica_amd.runtime.engines``; an experiment may override it.
-    engine = "plain"
+    #: Step engine: ``"auto"`` (the fused LDS-tile engine when the model
+    #: qualifies, else ``"plain"``), ``"fused"``, ``"plain"`` (the
+    #: reference's structure); an experiment may override it.
+    engine = "auto"
     #: Floating-point contraction of the generated code: ``"off"`` keeps
     #: every ``a * b + c`` as two roundings, ``"fast"`` lets the compiler
     #: fuse them (the default of the reference's nvcc); an experiment may
