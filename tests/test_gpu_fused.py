@@ -137,3 +137,24 @@ def test_batch_graph_replays():
         graphed.scatter()
         for a, b in zip(eager.models, graphed.models):
             assert crc(a) == crc(b)
+
+
+def test_batch_full_workload_subset_vs_plain():
+    """The 256-instance batch, 1000 steps; 24 instances (the largest and a
+    random draw) checked against the plain engine run alone."""
+    exps = batch_experiments()
+    models = [LifeLikeRule(exp) for exp in exps]
+    batch = Batch(models)
+    batch.run(1000)
+    batch.synchronize()
+    batch.scatter()
+    sizes = [exp.size[0] for exp in exps]
+    rng = np.random.default_rng(11)
+    picks = {int(np.argmax(sizes)), int(np.argmin(sizes))}
+    picks |= set(int(x) for x in rng.choice(len(exps), 22, replace=False))
+    for idx in sorted(picks):
+        plain_exp = type("Plain%d" % idx, (exps[idx],), {"engine": "plain"})
+        single = LifeLikeRule(plain_exp)
+        single.rule = exps[idx].rule
+        run(single, 1000)
+        assert crc(models[idx]) == crc(single), "instance %d" % idx
