@@ -37,6 +37,11 @@ def crcs(model):
 
 
 def test_genetics_general():
+    # parameters are class-level (xentica/core/parameters.py:143-145): start
+    # from the defaults, as the reference's test does in a fresh process
+    for name, value in (("death_speed", 15), ("max_genes", 9),
+                        ("mutation_prob", 0.0)):
+        EvoLife.__dict__[name].__set__(None, value)
     model = run(EvoLife(CrossbreedingExperiment, legacy_coloring=False), 100)
     cells, colors = crcs(model)
     assert cells == 2061779088

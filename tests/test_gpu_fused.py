@@ -54,6 +54,9 @@ def test_golden(cls, exp, steps, want):
 
 def test_evolife_golden():
     """``test_base.py:197-216`` on the fused engine (uint64 words)."""
+    for name, value in (("death_speed", 15), ("max_genes", 9),
+                        ("mutation_prob", 0.0)):
+        EvoLife.__dict__[name].__set__(None, value)
     model = run(EvoLife(fused(CrossbreedingExperiment),
                         legacy_coloring=False), 100)
     assert crc(model) == 2061779088
