@@ -102,3 +102,77 @@ def batch_experiments(count=256, seed=42, side_min=64, side_max=2048):
         experiments.append(type("BatchExperiment%d" % idx,
                                 (core.Experiment,), attrs))
     return experiments
+
+
+class LifeLike3D(core.CellularAutomaton):
+    """
+    Life-like rule on a 3-D Moore torus (26 offsets, the reference's list).
+
+    ``born``/``sustain`` are count masks (bit c: c counted cells); the
+    count follows the reference's 3-D ``MooreNeighborhood``, which keeps the
+    centre and drops (1, 1, 1).
+    """
+
+    state = core.IntegerProperty(max_val=1)
+    born = core.Parameter(default=1 << 5, interactive=True)
+    sustain = core.Parameter(default=(1 << 4) | (1 << 5), interactive=True)
+
+    class Topology:
+        """3-D Moore torus."""
+
+        dimensions = 3
+        lattice = core.OrthogonalLattice()
+        neighborhood = core.MooreNeighborhood()
+        border = core.TorusBorder()
+
+    def emit(self):
+        """Every buffer carries the state."""
+        for buf in self.buffers:
+            buf.state = self.main.state
+
+    def absorb(self):
+        """Apply the masks to the count."""
+        count = core.IntegerVariable()
+        for i in range(len(self.buffers)):
+            count += self.neighbors[i].buffer.state
+        is_born = (self.born >> count) & 1
+        is_sustain = (self.sustain >> count) & 1
+        self.main.state = is_born | is_sustain & self.main.state
+
+    @color_effects.MovingAverage
+    def color(self):
+        """Live cells white (projected by the renderer)."""
+        val = self.main.state * 255
+        return (val, val, val)
+
+
+class Life3DExperiment(core.Experiment):
+    """B5/S45 at 10 % density (``(RandInt(0, 9) + 7) >> 4``), word 42."""
+
+    word = 42
+    size = (1024, 1024, 1024)
+    zoom = 1
+    pos = [0, 0, 0]
+    born = 1 << 5
+    sustain = (1 << 4) | (1 << 5)
+    seed = seeds.patterns.PrimordialSoup(
+        vals={"state": (seeds.random.RandInt(0, 9) + 7) >> 4})
+
+
+def life3d_cells(size=(1024, 1024, 1024), word=42, chunk=1 << 26):
+    """
+    Step-0 cells of :class:`Life3DExperiment` at ``size``, drawn in chunks.
+
+    Identical to ``PrimordialSoup({'state': (RandInt(0, 9) + 7) >> 4})``:
+    the legacy ``RandomState.randint`` stream continues across calls, so
+    chunked draws equal one draw of the whole field.
+    """
+    import numpy as np
+    from xentica_amd.seeds.random import LocalRandom
+    stream = LocalRandom(word).numpy
+    n = int(np.prod(size))
+    cells = np.empty(n, dtype=np.uint8)
+    for start in range(0, n, chunk):
+        count = min(chunk, n - start)
+        cells[start:start + count] = (stream.randint(0, 10, count) + 7) >> 4
+    return cells

@@ -42,6 +42,20 @@ int xen_life_bits_run(void *buf_a, void *buf_b, int width_words, int height,
                       uint32_t born, uint32_t sustain, long long steps, int depth,
                       int strips, int *result_in_b, xen_stream stream);
 
+/* ---- 3-D Life-like engines (torus, the reference's 3-D Moore list) ----
+ * The reference's 3-D MooreNeighborhood keeps the centre and drops (1,1,1)
+ * (xentica/core/topology/neighborhood.py:159-167); the count of a cell is
+ * its 3x3x3 cube minus the cell at (x+1, y+1, z+1) and
+ * next = ((born >> count) & 1) | ((sustain >> count) & 1) & state.
+ * Bit-packed: 32 cells per uint32, x fastest, width / 32 dividing 64,
+ * field < 2 GiB; z_tiles = 0 picks about four waves per SIMD.  One call
+ * advances one step from src to dst. */
+int xen_life3d_bits_step(const void *src, void *dst, int width, int height, int depth,
+                         uint32_t born, uint32_t sustain, int z_tiles, xen_stream stream);
+/* Same rule on one uint8 per cell (a direct, unpacked restatement). */
+int xen_life3d_u8_step(const void *src, void *dst, int width, int height, int depth,
+                       uint32_t born, uint32_t sustain, xen_stream stream);
+
 /* ---- layout conversion between the reference plane and packed bits ----
  * cells: the reference's main-state plane (one uint8 per cell, value 0/1,
  * xentica/core/base.py:659-661); bits: 32 cells per uint32, x0 fastest.

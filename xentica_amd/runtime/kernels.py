@@ -12,7 +12,8 @@ from xentica_amd.runtime import hip
 from xentica_amd.runtime.crc import combine_partials
 
 __all__ = [
-    'life_bits_launch', 'life_bits_run', 'bits_pack', 'bits_unpack',
+    'life_bits_launch', 'life_bits_run', 'life3d_bits_step',
+    'life3d_u8_step', 'bits_pack', 'bits_unpack',
     'crc32_u8', 'crc32_bits', 'CRC_CHUNK',
 ]
 
@@ -25,6 +26,8 @@ _SIGNATURES = {
     'xen_life_bits_launch': [_vp, _vp, _i, _i, _u32, _u32, _i, _i, _vp],
     'xen_life_bits_run': [_vp, _vp, _i, _i, _u32, _u32, _ll, _i, _i,
                           ctypes.POINTER(_i), _vp],
+    'xen_life3d_bits_step': [_vp, _vp, _i, _i, _i, _u32, _u32, _i, _vp],
+    'xen_life3d_u8_step': [_vp, _vp, _i, _i, _i, _u32, _u32, _vp],
     'xen_bits_pack': [_vp, _vp, _ll, _vp],
     'xen_bits_unpack': [_vp, _vp, _ll, _vp],
     'xen_crc32_u8_partials': [_vp, _ll, _ll, _vp, _vp],
@@ -68,6 +71,20 @@ def life_bits_run(buf_a, buf_b, width_words, height, born, sustain, steps,
         buf_a, buf_b, width_words, height, born, sustain, steps, depth,
         strips, ctypes.byref(in_b), _s(stream)), "xen_life_bits_run")
     return bool(in_b.value)
+
+
+def life3d_bits_step(src, dst, size, born, sustain, z_tiles=0, stream=None):
+    """One step of a packed 3-D Life-like torus of ``size = (w, h, d)``."""
+    hip.check(_lib().xen_life3d_bits_step(
+        src, dst, size[0], size[1], size[2], born, sustain, z_tiles,
+        _s(stream)), "xen_life3d_bits_step")
+
+
+def life3d_u8_step(src, dst, size, born, sustain, stream=None):
+    """One step of a uint8 3-D Life-like torus (direct stencil)."""
+    hip.check(_lib().xen_life3d_u8_step(
+        src, dst, size[0], size[1], size[2], born, sustain, _s(stream)),
+        "xen_life3d_u8_step")
 
 
 def bits_pack(cells, bits, n_cells, stream=None):

@@ -14,7 +14,7 @@ import numpy as np
 from xentica_amd.core.exceptions import XenticaException
 from xentica_amd.runtime import hip, kernels
 
-__all__ = ['LifeBitsField', ]
+__all__ = ['LifeBitsField', 'Life3DField', 'Life3DBytes', ]
 
 
 class LifeBitsField:
@@ -109,6 +109,123 @@ class LifeBitsField:
             self.height, self.born, self.sustain, steps, self.depth,
             self.strips, self.stream)
         if in_b:
+            self._cur = 1 - self._cur
+
+    def synchronize(self):
+        """Wait for queued work."""
+        if self.stream is not None:
+            self.stream.synchronize()
+        else:
+            hip.device().synchronize()
+
+
+class Life3DField:
+    """
+    A packed 3-D Life-like torus with the reference's 3-D Moore list.
+
+    The count of a cell is its 3x3x3 cube minus the cell at (x+1, y+1,
+    z+1): the reference's ``MooreNeighborhood`` in 3-D keeps the centre and
+    drops that offset (``xentica/core/topology/neighborhood.py:159-167``).
+    32 cells per 32-bit word, x fastest; ``width / 32`` must divide 64.
+    """
+
+    def __init__(self, size, born, sustain, z_tiles=0, stream=None):
+        width, height, depth = (int(v) for v in size)
+        if width % 32 or 64 % (width // 32):
+            raise XenticaException("width / 32 must divide 64")
+        self.size = (width, height, depth)
+        self.n_cells = width * height * depth
+        self.born, self.sustain = int(born), int(sustain)
+        self.z_tiles = int(z_tiles)
+        self.stream = stream
+        self._bufs = [hip.Buffer(self.n_cells // 8), hip.Buffer(self.n_cells // 8)]
+        self._cur = 0
+
+    @property
+    def state_ptr(self):
+        """Device address of the current state."""
+        return self._bufs[self._cur].ptr
+
+    def set_cells(self, cells, chunk=1 << 28):
+        """Load a uint8 plane (x fastest), staged in chunks."""
+        cells = np.ascontiguousarray(cells, dtype=np.uint8).reshape(-1)
+        if cells.size != self.n_cells:
+            raise XenticaException("expected %d cells" % self.n_cells)
+        staging = hip.Buffer(min(chunk, self.n_cells))
+        for start in range(0, self.n_cells, chunk):
+            part = cells[start:start + chunk] & 1
+            staging.upload(part, self.stream)
+            kernels.bits_pack(staging.ptr, self.state_ptr + start // 8,
+                              part.size, self.stream)
+            self.synchronize()
+
+    def get_cells(self):
+        """The state as a uint8 plane."""
+        staging = hip.Buffer(self.n_cells)
+        kernels.bits_unpack(self.state_ptr, staging.ptr, self.n_cells,
+                            self.stream)
+        out = np.empty(self.n_cells, dtype=np.uint8)
+        staging.download(out, self.stream)
+        return out
+
+    def crc32(self):
+        """``binascii.crc32`` of the uint8 plane, on the device."""
+        return kernels.crc32_bits(self.state_ptr, self.n_cells, self.stream)
+
+    def step(self, steps=1):
+        """Advance ``steps`` steps (asynchronous)."""
+        for _ in range(steps):
+            kernels.life3d_bits_step(
+                self.state_ptr, self._bufs[1 - self._cur].ptr, self.size,
+                self.born, self.sustain, self.z_tiles, self.stream)
+            self._cur = 1 - self._cur
+
+    def synchronize(self):
+        """Wait for queued work."""
+        if self.stream is not None:
+            self.stream.synchronize()
+        else:
+            hip.device().synchronize()
+
+
+class Life3DBytes:
+    """The same 3-D rule on one uint8 per cell (the unpacked restatement)."""
+
+    def __init__(self, size, born, sustain, stream=None):
+        self.size = tuple(int(v) for v in size)
+        self.n_cells = self.size[0] * self.size[1] * self.size[2]
+        self.born, self.sustain = int(born), int(sustain)
+        self.stream = stream
+        self._bufs = [hip.Buffer(self.n_cells), hip.Buffer(self.n_cells)]
+        self._cur = 0
+
+    @property
+    def state_ptr(self):
+        """Device address of the current state."""
+        return self._bufs[self._cur].ptr
+
+    def set_cells(self, cells):
+        """Load a uint8 plane."""
+        self._bufs[self._cur].upload(
+            np.ascontiguousarray(cells, dtype=np.uint8).reshape(-1) & 1,
+            self.stream)
+
+    def get_cells(self):
+        """The state as a uint8 plane."""
+        out = np.empty(self.n_cells, dtype=np.uint8)
+        self._bufs[self._cur].download(out, self.stream)
+        return out
+
+    def crc32(self):
+        """``binascii.crc32`` of the plane, on the device."""
+        return kernels.crc32_u8(self.state_ptr, self.n_cells, self.stream)
+
+    def step(self, steps=1):
+        """Advance ``steps`` steps (asynchronous)."""
+        for _ in range(steps):
+            kernels.life3d_u8_step(self.state_ptr, self._bufs[1 - self._cur].ptr,
+                                   self.size, self.born, self.sustain,
+                                   self.stream)
             self._cur = 1 - self._cur
 
     def synchronize(self):
