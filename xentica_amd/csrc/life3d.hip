@@ -36,9 +36,12 @@ template <uint8_t kTable>
 __device__ __forceinline__ uint32_t op3(uint32_t a, uint32_t b, uint32_t c) {
     return __builtin_amdgcn_bitop3_b32(a, b, c, kTable);
 }
-constexpr uint8_t kXor3 = 0x96, kMaj3 = 0xE8, kMux = 0xCA;
-constexpr uint8_t kNotAAndBAndNotC = 0x04;  // ~a & b & ~c
-constexpr uint8_t kAAndNotBAndC = 0x22;     // a & ~b & c
+#include "bitop3.h"
+constexpr uint8_t kXor3 = bitop3_table([](bool a, bool b, bool c) { return a ^ b ^ c; });
+constexpr uint8_t kMaj3 = bitop3_table([](bool a, bool b, bool c) { return (a + b + c) >= 2; });
+constexpr uint8_t kMux = bitop3_table([](bool a, bool b, bool c) { return a ? b : c; });
+constexpr uint8_t kNotAAndBAndNotC = bitop3_table([](bool a, bool b, bool c) { return !a && b && !c; });
+constexpr uint8_t kAAndNotBAndC = bitop3_table([](bool a, bool b, bool c) { return a && !b && c; });
 
 __device__ __forceinline__ int wrap(int v, int n) {
     v %= n;

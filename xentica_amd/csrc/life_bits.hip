@@ -25,6 +25,7 @@
 #include "xentica_hip.h"
 #include "xentica_kernels.h"
 #include "common.h"
+#include "bitop3.h"
 
 namespace {
 
@@ -55,11 +56,11 @@ template <uint8_t kTable>
 __device__ __forceinline__ uint32_t bitop3(uint32_t a, uint32_t b, uint32_t c) {
     return __builtin_amdgcn_bitop3_b32(a, b, c, kTable);
 }
-constexpr uint8_t kXor3 = 0x96;  // a ^ b ^ c
-constexpr uint8_t kMaj3 = 0xE8;  // at least two of a, b, c
-constexpr uint8_t kMux = 0xCA;   // a ? b : c
-constexpr uint8_t kXorAndNot = 0x14;  // (a ^ b) & ~c
-constexpr uint8_t kAndOr = 0xE0;      // a & (b | c)
+constexpr uint8_t kXor3 = bitop3_table([](bool a, bool b, bool c) { return a ^ b ^ c; });
+constexpr uint8_t kMaj3 = bitop3_table([](bool a, bool b, bool c) { return (a + b + c) >= 2; });
+constexpr uint8_t kMux = bitop3_table([](bool a, bool b, bool c) { return a ? b : c; });
+constexpr uint8_t kXorAndNot = bitop3_table([](bool a, bool b, bool c) { return (a ^ b) && !c; });
+constexpr uint8_t kAndOr = bitop3_table([](bool a, bool b, bool c) { return a && (b || c); });
 
 // Rule application.  The neighbour count is count = c0 + 2 * s2 with
 // s2 = x + k0 + 2y (x, k0, y single bits); kFast selects B3/S23, where
