@@ -183,6 +183,41 @@ def batched(steps=1000, plain_steps=10):
     }
 
 
+def life3d(steps=200):
+    """3-D B5/S45, 1024^3 torus, 10 % density: packed vs unpacked engines."""
+    from examples.workloads import life3d_cells
+    from xentica_amd.runtime import hip
+    from xentica_amd.runtime.lifebits import Life3DField, Life3DBytes
+    size = (1024, 1024, 1024)
+    n = size[0] * size[1] * size[2]
+    born, sustain = 1 << 5, (1 << 4) | (1 << 5)
+    cells = life3d_cells(size)
+    stream = hip.Stream()
+    out = {"workload": "3-D Life-like B5/S45, reference 26-offset Moore list, "
+                       "1024^3 torus, 10 % density, %d steps" % steps}
+    for name, cls, bpc in (("bitpacked", Life3DField, 0.25),
+                           ("uint8", Life3DBytes, 2.0)):
+        field = cls(size, born, sustain, stream=stream)
+        field.set_cells(cells)
+        field.step(2)
+        start, stop = hip.Event(), hip.Event()
+        stream.synchronize()
+        start.record(stream)
+        field.step(steps)
+        stop.record(stream)
+        stop.synchronize()
+        ms = start.elapsed_ms(stop)
+        value = n * steps / (ms / 1e3)
+        out[name] = {"value": value, "unit": "cell-updates/s",
+                     "ms_per_step": ms / steps,
+                     "algorithmic_GBps": value * bpc / 1e9,
+                     "roofline_frac": value * bpc / 1e9 / HBM_PEAK_GBS}
+        del field
+    out["uint8_layout_roofline_frac_of_packed"] = (
+        out["bitpacked"]["value"] * 2.0 / 1e9 / HBM_PEAK_GBS)
+    return out
+
+
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -193,6 +228,7 @@ def main():
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--no-plain", action="store_true")
     parser.add_argument("--no-batch", action="store_true")
+    parser.add_argument("--no-3d", action="store_true")
     args = parser.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -256,6 +292,9 @@ def main():
         log("plain engine ...")
         result["plain_engine"] = plain_engine()
         result["speedup_vs_plain"] = value / result["plain_engine"]["value"]
+    if not args.no_3d:
+        log("3-D workload ...")
+        result["life3d"] = life3d()
     if not args.no_batch:
         log("batched workload ...")
         result["batched"] = batched()
