@@ -194,7 +194,7 @@ def life3d(steps=200):
     cells = life3d_cells(size)
     stream = hip.Stream()
     out = {"workload": "3-D Life-like B5/S45, reference 26-offset Moore list, "
-                       "1024^3 torus, 10 % density, %d steps" % steps}
+                       "1024^3 torus, 10 %% density, %d steps" % steps}
     for name, cls, bpc in (("bitpacked", Life3DField, 0.25),
                            ("uint8", Life3DBytes, 2.0)):
         field = cls(size, born, sustain, stream=stream)

@@ -340,6 +340,8 @@ class Translator:
 
     def pack_state(self, state):
         """Pack ``{name: value}`` into container words, without masking."""
+        if self.main.dtype is UINT128:
+            return _pack_wide(self.main.properties, state)
         val = 0
         shift = 0
         for name, prop in self.main.properties.items():
@@ -347,6 +349,40 @@ class Translator:
                 val += state[name] << shift
             shift += prop.bit_width
         return val
+
+
+def _pack_wide(properties, state):
+    """
+    :meth:`pack_state` for 128-bit words: ``sum(value << shift)`` modulo
+    2**128 with each value sign-extended from int64, as the 64-bit path
+    computes modulo 2**64; returns a :data:`UINT128` array.
+    """
+    size = max(np.size(val) for val in state.values()) if state else 1
+    lo = np.zeros(size, dtype=np.uint64)
+    hi = np.zeros(size, dtype=np.uint64)
+    shift = 0
+    with np.errstate(over='ignore'):
+        for name, prop in properties.items():
+            if name in state:
+                val = np.broadcast_to(
+                    np.asarray(state[name], dtype=np.int64), (size, ))
+                v_lo = val.astype(np.uint64)
+                v_hi = np.where(val < 0, np.uint64(2 ** 64 - 1),
+                                np.uint64(0))
+                if 0 < shift < 64:
+                    v_hi = (v_hi << np.uint64(shift)) | (
+                        v_lo >> np.uint64(64 - shift))
+                    v_lo = v_lo << np.uint64(shift)
+                elif shift >= 64:
+                    v_hi = v_lo << np.uint64(shift - 64)
+                    v_lo = np.zeros_like(v_lo)
+                new_lo = lo + v_lo
+                hi = hi + v_hi + (new_lo < lo).astype(np.uint64)
+                lo = new_lo
+            shift += prop.bit_width
+    out = np.empty(size, dtype=UINT128)
+    out['lo'], out['hi'] = lo, hi
+    return out
 
 
 class GPUKernels:

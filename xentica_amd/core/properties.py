@@ -18,6 +18,13 @@ Behaviour follows ``xentica/core/properties.py``:
 * :class:`TotalisticRuleProperty` holds birth/survival bit masks
   (``properties.py:544-619``).
 
+Beyond the reference: a container wider than 64 bits is one
+``unsigned __int128`` word (host dtype :data:`UINT128`: two little-endian
+uint64 halves, low half first), so a model whose fields sum to 65-128
+bits keeps the one-word semantics above; and :class:`XorShiftProperty`
+is a 32-bit xorshift generator with :class:`RandomProperty`'s
+advance-on-access behaviour.
+
 Where a container's word lives in memory is asked of the model's
 addressing scheme (``bsca.addressing``): the plain scheme renders the
 reference's ``fld[i + n * plane]`` and neighbour-coordinate code
@@ -37,8 +44,13 @@ from xentica_amd.tools import xmath
 
 __all__ = [
     'Property', 'IntegerProperty', 'FloatProperty', 'ContainerProperty',
-    'TotalisticRuleProperty', 'RandomProperty',
+    'TotalisticRuleProperty', 'RandomProperty', 'XorShiftProperty',
+    'UINT128',
 ]
+
+#: Host dtype of a 128-bit container word: the little-endian byte image
+#: of ``unsigned __int128`` (low 64 bits first).
+UINT128 = np.dtype([('lo', '<u8'), ('hi', '<u8')])
 
 #: (bit width, NumPy dtype, C type stem), smallest first.
 _TYPES = (
@@ -46,6 +58,7 @@ _TYPES = (
     (16, np.uint16, 'short'),
     (32, np.uint32, 'int'),
     (64, np.uint64, 'long'),
+    (128, UINT128, '__int128'),
 )
 
 
@@ -437,3 +450,37 @@ class RandomProperty(Property):
     def uniform(self):
         """The current value as a float in [0, 1]."""
         return xmath.float(DeferredExpression(self.var_name)) / 65535
+
+
+class XorShiftProperty(Property):
+    """
+    A 32-bit xorshift generator that advances on every access.
+
+    No counterpart in the reference, whose only generator is the LCG of
+    :class:`RandomProperty` (``properties.py:622-660``); the access
+    semantics are the same: each ``x.rng`` read emits one advance and
+    schedules the container's store, ``x.rng.uniform`` is the value that
+    access produced as a float in [0, 1].  The advance is Marsaglia's
+    xorshift with shifts (13, 17, 5) (G. Marsaglia, "Xorshift RNGs",
+    Journal of Statistical Software 8(14), 2003): period 2**32 - 1 over
+    the non-zero states, so seed every cell with a non-zero value.
+    """
+
+    def __get__(self, obj, objtype):
+        """Advance the generator, then return the property."""
+        self.declare_once()
+        var = self.var_name
+        for c_op, shift in (("<<", 13), (">>", 17), ("<<", 5)):
+            self.__set__(self, DeferredExpression(
+                "(%s ^ (%s %s %d))" % (var, var, c_op, shift)))
+        self.bsca.deferred_write(obj)
+        return self
+
+    def calc_bit_width(self):
+        """Always 32 bits."""
+        return 32
+
+    @property
+    def uniform(self):
+        """The current value as a float in [0, 1]."""
+        return xmath.float(DeferredExpression(self.var_name)) / 4294967295
