@@ -26,6 +26,11 @@
  *   oracle_color_life    GameOfLifeColor (examples/game_of_life.py:112-159)
  *   oracle_shifting_sands ShiftingSands (examples/shifting_sands.py:15-47)
  *   oracle_noisetv       NoiseTV (examples/noisetv.py:10-28)
+ *   oracle_energy_life   EnergyLife (no reference counterpart: the
+ *                        project's EvoLife-style model on a 128-bit word,
+ *                        written in the reference's DSL; the rule is
+ *                        restated here from its definition)
+ *   oracle_xorshift32_period  period of the xorshift32 map from a state
  *   oracle_crc32         CRC-32 of binascii.crc32 (tests/core/test_base.py:39-41)
  *
  * Build: see oracle/Makefile (gcc -O3 -fopenmp -ffp-contract=off).
@@ -277,6 +282,106 @@ int oracle_noisetv(uint16_t *fld, int32_t *col, int64_t n,
             }
             fld[i] = (uint16_t)(r & 65535u);
         }
+    }
+    return 0;
+}
+
+/*
+ * EnergyLife on a 128-bit word: energy (16 bits at 0), genome (18 bits at
+ * 16: birth mask bits 0-8, survival mask bits 9-17), mode (1 bit at 34),
+ * xorshift32 state (32 bits at 35).  Emit publishes energy, genome and
+ * mode (generator field 0) in every buffer slot.  Absorb, for each
+ * neighbour j with energy ej and mode mj: the energy moves by
+ * sign(ej - e) * (|ej - e| >> (3 + m + mj)); count = number of live
+ * neighbours; the richest live neighbour (first on ties, energy > 0) is
+ * the parent.  The generator advances once (x ^= x << 13; x ^= x >> 17;
+ * x ^= x << 5).  A live cell stays alive iff its survival bit for count
+ * is set and its new energy >= sustain; a dead cell becomes alive iff
+ * the parent's birth bit for count is set and its new energy >= birth;
+ * a newborn takes the parent's genome, with bit ((r >> 16) % 18) flipped
+ * when (r & 65535) < mutation.  Colour: MovingAverage of (mode * 255,
+ * energy, (genome >> 10) & 255) of the new state.
+ */
+typedef unsigned __int128 u128;
+
+static inline u128 el_pack(uint32_t e, uint32_t g, uint32_t m, uint32_t r) {
+    return (u128)(e & 65535u) | ((u128)(g & 262143u) << 16) |
+           ((u128)(m & 1u) << 34) | ((u128)r << 35);
+}
+
+int oracle_energy_life(u128 *fld, int32_t *col, int64_t w0, int64_t w1,
+                       const int8_t *delta, const int32_t *rev,
+                       uint32_t sustain, uint32_t birth, uint32_t mutation,
+                       int fade_in, int fade_out, int smooth, int steps,
+                       int threads, uint32_t *hashes) {
+    lattice_t lat;
+    int64_t sizes[2] = {w0, w1};
+    if (setup(&lat, 2, sizes, 8, delta, rev, 0, 0)) return 1;
+    set_threads(threads);
+    const int64_t n = lat.n;
+    for (int s = 0; s < steps; ++s) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            u128 w = fld[i];
+            u128 b = el_pack((uint32_t)(w & 65535u), (uint32_t)((w >> 16) & 262143u),
+                             (uint32_t)((w >> 34) & 1u), 0);
+            for (int q = 0; q < 8; ++q) fld[i + n * (rev[q] + 1)] = b;
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            u128 w = fld[i];
+            uint32_t e = (uint32_t)(w & 65535u);
+            uint32_t g = (uint32_t)((w >> 16) & 262143u);
+            uint32_t m = (uint32_t)((w >> 34) & 1u);
+            uint32_t r = (uint32_t)(w >> 35);
+            uint32_t ne = e, count = 0, best_e = 0, best_g = 0;
+            for (int j = 0; j < 8; ++j) {
+                u128 v = fld[neighbor_index(&lat, i, j) + n * (j + 1)];
+                uint32_t ej = (uint32_t)(v & 65535u);
+                uint32_t gj = (uint32_t)((v >> 16) & 262143u);
+                uint32_t mj = (uint32_t)((v >> 34) & 1u);
+                uint32_t sh = 3 + m + mj;
+                if (ej > e) ne += (ej - e) >> sh;
+                if (e > ej) ne -= (e - ej) >> sh;
+                count += mj;
+                if (mj && ej > best_e) {
+                    best_e = ej;
+                    best_g = gj;
+                }
+            }
+            r ^= r << 13;
+            r ^= r >> 17;
+            r ^= r << 5;
+            uint32_t nm;
+            if (m)
+                nm = ((g >> (9 + count)) & 1u) && ne >= sustain;
+            else
+                nm = ((best_g >> count) & 1u) && best_e > 0 && ne >= birth;
+            if (!m && nm) {
+                g = best_g;
+                if ((r & 65535u) < mutation) g ^= 1u << ((r >> 16) % 18u);
+            }
+            if (col) {
+                col[3 * i + 0] = fade((int32_t)(nm * 255), col[3 * i + 0], fade_in, fade_out, smooth);
+                col[3 * i + 1] = fade((int32_t)(ne & 65535u), col[3 * i + 1], fade_in, fade_out, smooth);
+                col[3 * i + 2] = fade((int32_t)((g >> 10) & 255u), col[3 * i + 2], fade_in, fade_out, smooth);
+            }
+            fld[i] = el_pack(ne, g, nm, r);
+        }
+        if (hashes) hashes[s] = oracle_crc32(0, (const uint8_t *)fld, n * 16);
+    }
+    return 0;
+}
+
+/* Number of xorshift32 (13, 17, 5) steps until the state returns to
+ * seed, or 0 if it does not within 2**32 steps. */
+uint64_t oracle_xorshift32_period(uint32_t seed) {
+    uint32_t x = seed;
+    for (uint64_t t = 1; t <= 4294967296ull; ++t) {
+        x ^= x << 13;
+        x ^= x >> 17;
+        x ^= x << 5;
+        if (x == seed) return t;
     }
     return 0;
 }

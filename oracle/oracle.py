@@ -30,6 +30,12 @@ Pinning (what checks each part; see ``tests/test_oracle.py``):
   sands      -- golden CRC 1639048270 and the closed form (roll by -1,-1)
   noisetv    -- golden CRC 2773894957 and the closed form (LCG twice)
   crc32      -- ``zlib.crc32``
+  energy_life -- no reference vector (the model has no reference
+                counterpart): pinned by exact energy conservation, the
+                min/max principle of its flux, single-spike closed forms,
+                reduction to B3/S23 Life (uniform genome, zero thresholds,
+                no mutation) checked against the NumPy roll-sum Life, and
+                the xorshift32 period 2**32 - 1 (Marsaglia 2003)
   3-D Moore quirk -- parity unpinned by any reference vector; pinned by
                 the offset list read from ``neighborhood.py:165-167`` and a
                 brute-force 3-D check in the tests.
@@ -46,8 +52,12 @@ __all__ = [
     'library', 'local_random', 'moore_deltas', 'von_neumann_deltas',
     'moore_rev', 'von_neumann_rev', 'big_bang', 'primordial_soup',
     'pack', 'Field', 'lifelike', 'color_life', 'shifting_sands', 'noisetv',
-    'crc32', 'golly2int', 'numpy_life_step',
+    'crc32', 'golly2int', 'numpy_life_step', 'WIDE', 'energy_life',
+    'energy_life_field', 'xorshift32_period', 'energy_fields',
 ]
+
+#: 128-bit little-endian word (low 64 bits first).
+WIDE = np.dtype([('lo', '<u8'), ('hi', '<u8')])
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -71,6 +81,11 @@ def library():
         lib.oracle_shifting_sands.argtypes = [
             p, p, i64, i64, p, p, i32, i32, i32, i32, i32]
         lib.oracle_noisetv.argtypes = [p, p, i64, i32, i32, i32, i32, i32]
+        lib.oracle_energy_life.argtypes = [
+            p, p, i64, i64, p, p, u32, u32, u32, i32, i32, i32, i32, i32, p]
+        lib.oracle_energy_life.restype = i32
+        lib.oracle_xorshift32_period.argtypes = [u32]
+        lib.oracle_xorshift32_period.restype = ctypes.c_uint64
         lib.oracle_crc32.argtypes = [u32, p, i64]
         lib.oracle_crc32.restype = u32
         for name in ('oracle_lifelike', 'oracle_color_life',
@@ -238,6 +253,64 @@ def noisetv(field, steps, fade=(255, 255, 1), threads=0):
     assert library().oracle_noisetv(
         _ptr(field.fld), _ptr(field.col), field.n, fade[0], fade[1],
         fade[2], steps, threads) == 0
+
+
+def energy_life(field, steps, sustain=32, birth=128, mutation=16,
+                fade=(255, 255, 1), colors=True, threads=0, hashes=False):
+    """EnergyLife steps in place on a :data:`WIDE` field."""
+    dl = np.array(moore_deltas(2), dtype=np.int8).reshape(-1)
+    rv = np.array(moore_rev(2), dtype=np.int32)
+    out = np.zeros(max(steps, 1), dtype=np.uint32) if hashes else None
+    assert field.fld.dtype == WIDE
+    assert library().oracle_energy_life(
+        _ptr(field.fld), _ptr(field.col) if colors else None,
+        field.size[0], field.size[1], _ptr(dl), _ptr(rv), sustain, birth,
+        mutation, fade[0], fade[1], fade[2], steps, threads, _ptr(out)) == 0
+    return out[:steps] if hashes else None
+
+
+def xorshift32_period(seed):
+    """Steps until xorshift32 (13, 17, 5) returns to ``seed`` (0: never)."""
+    return int(library().oracle_xorshift32_period(seed))
+
+
+def energy_fields(words):
+    """``(energy, genome, mode, rng)`` arrays of :data:`WIDE` words."""
+    lo = words['lo'].astype(np.uint64)
+    hi = words['hi'].astype(np.uint64)
+    energy = (lo & np.uint64(0xFFFF)).astype(np.int64)
+    genome = ((lo >> np.uint64(16)) & np.uint64(0x3FFFF)).astype(np.int64)
+    mode = ((lo >> np.uint64(34)) & np.uint64(1)).astype(np.int64)
+    rng = ((lo >> np.uint64(35)) | (hi << np.uint64(29))) & np.uint64(
+        0xFFFFFFFF)
+    return energy, genome, mode, rng.astype(np.int64)
+
+
+def energy_words(energy, genome, mode, rng):
+    """Pack non-negative in-range fields into :data:`WIDE` words."""
+    energy, genome, mode, rng = (np.asarray(v, dtype=np.uint64)
+                                 for v in (energy, genome, mode, rng))
+    out = np.zeros(energy.shape, dtype=WIDE)
+    out['lo'] = (energy | (genome << np.uint64(16)) | (mode << np.uint64(34))
+                 | (rng << np.uint64(35)))
+    out['hi'] = rng >> np.uint64(29)
+    return out
+
+
+def energy_life_field(size, word=42):
+    """
+    Step-0 state of ``EnergyLifeExperiment``: ``PrimordialSoup`` drawing,
+    in sorted name order, energy ``RandInt(0, 255)``, genome
+    ``RandInt(0, 2**18 - 1)``, mode ``RandInt(0, 1)`` and generator
+    ``RandInt(1, 2**32 - 1)``.
+    """
+    rng = local_random(word)
+    vals, _ = primordial_soup(size, {
+        "energy": randint(0, 255), "genome": randint(0, 2 ** 18 - 1),
+        "mode": randint(0, 1), "rng": randint(1, 2 ** 32 - 1)}, rng)
+    words = energy_words(vals["energy"], vals["genome"], vals["mode"],
+                         vals["rng"])
+    return Field(size, 8, WIDE, words)
 
 
 def crc32(data):

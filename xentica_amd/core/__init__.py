@@ -7,6 +7,7 @@ Shortcuts: ``core.CellularAutomaton``, ``core.Experiment``,
 ``core.VonNeumannNeighborhood``, ``core.TorusBorder``,
 ``core.StaticBorder``, ``core.IntegerProperty``, ``core.FloatProperty``,
 ``core.TotalisticRuleProperty``, ``core.RandomProperty``,
+``core.XorShiftProperty`` (no reference counterpart),
 ``core.Parameter``, ``core.IntegerVariable``, ``core.FloatVariable``.
 """
 from xentica_amd.core.base import CellularAutomaton
@@ -15,6 +16,7 @@ from xentica_amd.core.properties import (
     FloatProperty,
     TotalisticRuleProperty,
     RandomProperty,
+    XorShiftProperty,
 )
 from xentica_amd.core.variables import (
     IntegerVariable, FloatVariable,
@@ -39,6 +41,7 @@ __all__ = [
     'FloatProperty',
     'TotalisticRuleProperty',
     'RandomProperty',
+    'XorShiftProperty',
     'Parameter',
     'IntegerVariable',
     'FloatVariable',

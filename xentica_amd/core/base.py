@@ -30,7 +30,7 @@ from xentica_amd.bridge import MoireBridge
 from xentica_amd.seeds.random import LocalRandom
 from xentica_amd.core import _context
 from xentica_amd.core.addressing import PlainAddressing, TileAddressing
-from xentica_amd.core.properties import Property, ContainerProperty
+from xentica_amd.core.properties import Property, ContainerProperty, UINT128
 from xentica_amd.core.renderers import RendererPlain
 from xentica_amd.core.exceptions import XenticaException
 from xentica_amd.runtime import arrays as gpuarray

@@ -58,8 +58,11 @@ _TYPES = (
     (16, np.uint16, 'short'),
     (32, np.uint32, 'int'),
     (64, np.uint64, 'long'),
-    (128, UINT128, '__int128'),
 )
+#: The word of a container of 65-128 bits (containers only: a single
+#: property wider than 64 bits keeps the reference's 64-bit type and a
+#: ``width`` of several words, ``properties.py:82-99``).
+_WIDE_TYPE = (128, UINT128, '__int128')
 
 
 class Property(DeferredExpression, BscaDetectorMixin):
@@ -268,6 +271,17 @@ class ContainerProperty(Property):
     def calc_bit_width(self):
         """Sum of the inner properties' bit widths."""
         return sum(p.bit_width for p in self._properties.values())
+
+    @cached_property
+    def best_type(self):
+        """The smallest word holding every field: up to 128 bits."""
+        if self.bit_width <= 64:
+            return super().best_type
+        if self.bit_width <= 128:
+            return _WIDE_TYPE
+        raise XenticaException(
+            "cell state of %d bits does not fit one 128-bit word"
+            % self.bit_width)
 
     @Property.buf_num.setter
     def buf_num(self, val):
