@@ -135,19 +135,20 @@ def batched(steps=1000, plain_steps=10):
     """
     The batch workload: 256 Life-like tori, sides 64-2048, own rules.
 
-    Batched: one persistent fused launch per 4 steps over every instance,
-    the 1000-step run captured in a HIP graph.  Baseline: the per-instance
+    Batched: one persistent fused launch per ``depth`` steps over every
+    instance (tile and depth chosen by the autotuner), the 1000-step run
+    captured in a HIP graph.  Baseline: the per-instance
     plain launch loop (two launches per instance per step), timed over
     ``plain_steps`` steps.
     """
     from examples.workloads import LifeLikeRule, batch_experiments
     from xentica_amd.runtime import hip
-    from xentica_amd.runtime.fused import Batch
+    from xentica_amd.runtime.fused import autotune
     exps = batch_experiments()
     models = [LifeLikeRule(exp) for exp in exps]
     total = sum(model.cells_num for model in models)
-    batch = Batch(models)
-    batch.run(steps)                       # compile, capture, warm up
+    batch, tuned = autotune(models, log=log)
+    batch.run(steps)                       # capture, warm up
     batch.synchronize()
     start, stop = hip.Event(), hip.Event()
     start.record(batch.stream)
@@ -178,6 +179,8 @@ def batched(steps=1000, plain_steps=10):
         "cells": total, "value": value, "unit": "cell-updates/s",
         "ms": ms, "tile": list(batch.program.tile),
         "depth": batch.program.depth,
+        "autotune": {"%dx%d/d%d" % (t[0], t[1], d): v
+                     for (t, d), v in tuned.items()},
         "plain_loop_value": plain_value, "plain_loop_steps": plain_steps,
         "speedup_vs_plain": value / plain_value,
     }
@@ -218,6 +221,61 @@ def life3d(steps=200):
     return out
 
 
+def energy_life(steps=5000, plain_steps=20):
+    """
+    EnergyLife on a 4096x4096 torus: 128-bit words (energy 16 bits, genome
+    18, mode 1, xorshift32 32), word 42, ``steps`` steps by the fused
+    engine in one graph-captured run; total energy checked before and
+    after; the plain emit/absorb launch loop timed over ``plain_steps``.
+    """
+    from examples.energy_life import EnergyLife, EnergyLifeExperiment
+    from oracle.oracle import energy_fields
+    from xentica_amd.runtime import hip
+    from xentica_amd.runtime.fused import autotune
+    model = EnergyLife(EnergyLifeExperiment)
+    n = model.cells_num
+    total0 = int(energy_fields(model.gpu.arrays.cells.get()[:n])[0].sum())
+    batch, _ = autotune([model], log=log)
+    batch.run(10)
+    batch.synchronize()
+    start, stop = hip.Event(), hip.Event()
+    start.record(batch.stream)
+    batch.run(steps)
+    stop.record(batch.stream)
+    stop.synchronize()
+    ms = start.elapsed_ms(stop)
+    batch.scatter()
+    energy, _, mode, _ = energy_fields(model.gpu.arrays.cells.get()[:n])
+    value = n * steps / (ms / 1e3)
+    plain_exp = type("EnergyLifePlain", (EnergyLifeExperiment, ),
+                     {"engine": "plain"})
+    plain = EnergyLife(plain_exp)
+    plain.step()
+    hip.device().synchronize()
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        plain.step()
+    hip.device().synchronize()
+    plain_value = n * plain_steps / (time.perf_counter() - t0)
+    bpc = 32.0
+    return {
+        "workload": "EnergyLife 4096x4096 torus, 128-bit words (energy 16, "
+                    "genome 18, mode 1, xorshift32), word 42, %d steps"
+                    % steps,
+        "value": value, "unit": "cell-updates/s", "ms_per_step": ms / steps,
+        "engine": "fused, tile %s depth %d, graph" % (
+            list(batch.program.tile), batch.program.depth),
+        "algorithmic_bytes_per_cell_update": bpc,
+        "algorithmic_GBps": value * bpc / 1e9,
+        "roofline_frac": value * bpc / 1e9 / HBM_PEAK_GBS,
+        "energy_total_start": total0, "energy_total_end": int(energy.sum()),
+        "energy_conserved": int(energy.sum()) == total0,
+        "alive_fraction_end": float(mode.mean()),
+        "plain_loop_value": plain_value, "plain_loop_steps": plain_steps,
+        "speedup_vs_plain": value / plain_value,
+    }
+
+
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -229,6 +287,7 @@ def main():
     parser.add_argument("--no-plain", action="store_true")
     parser.add_argument("--no-batch", action="store_true")
     parser.add_argument("--no-3d", action="store_true")
+    parser.add_argument("--no-energy", action="store_true")
     args = parser.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -295,6 +354,9 @@ def main():
     if not args.no_3d:
         log("3-D workload ...")
         result["life3d"] = life3d()
+    if not args.no_energy:
+        log("EnergyLife workload ...")
+        result["energy_life"] = energy_life()
     if not args.no_batch:
         log("batched workload ...")
         result["batched"] = batched()

@@ -1,0 +1,85 @@
+"""
+Emit folding of the fused kernels, decided at translation (CPU only).
+
+Buffer slot ``p`` folds to ``(main & keep) | const`` when emit stores only
+copies of the cell's own main fields and integer constants; anything else
+keeps the buffer planes.
+"""
+from xentica_amd import core
+from xentica_amd.runtime.fused import FusedProgram
+from examples.base import RegularCA
+from examples.game_of_life import GameOfLife, GOLExperiment
+from examples.shifting_sands import ShiftingSands, ShiftingSandsExperiment
+from examples.energy_life import EnergyLife, EnergyLifeSmall
+
+
+class Flipped(RegularCA):
+    """Emit stores a function of the state: not foldable."""
+
+    state = core.IntegerProperty(max_val=1)
+
+    def emit(self):
+        for buf in self.buffers:
+            buf.state = self.main.state ^ 1
+
+    def absorb(self):
+        count = core.IntegerVariable()
+        for i in range(len(self.buffers)):
+            count += self.neighbors[i].buffer.state
+        self.main.state = count & 1
+
+    def color(self):
+        return (0, 0, 0)
+
+
+class StaticTwo(GameOfLife):
+    """A static border whose word has a bit emit does not copy."""
+
+    marker = core.IntegerProperty(max_val=1)
+
+    class Topology(GameOfLife.Topology):
+        border = core.StaticBorder(3)
+
+    def emit(self):
+        for buf in self.buffers:
+            buf.state = self.main.state
+
+
+def _fold(cls, exp, **kwargs):
+    return cls(exp).translate_tile(**kwargs)[2].fold
+
+
+def test_life_folds_to_copies():
+    fold = _fold(GameOfLife, GOLExperiment)
+    assert fold == {slot: (1, 0) for slot in range(1, 9)}
+
+
+def test_shifting_sands_folds_to_one_copy_and_zeros():
+    fold = _fold(ShiftingSands, ShiftingSandsExperiment)
+    # buffers[0] goes to slot rev[0] + 1 = 8; the others carry 0
+    assert fold[8] == (1, 0)
+    assert all(fold[slot] == (0, 0) for slot in range(1, 8))
+
+
+def test_wide_word_fold_keeps_energy_genome_mode():
+    fold = _fold(EnergyLife, EnergyLifeSmall)
+    assert set(fold.values()) == {((1 << 35) - 1, 0)}
+    program = FusedProgram(EnergyLife(EnergyLifeSmall))
+    assert program.folded and program.halo_per_step == 1
+    assert "_tb" not in program.source
+
+
+def test_function_of_state_does_not_fold():
+    assert _fold(Flipped, GOLExperiment) is None
+    program = FusedProgram(Flipped(GOLExperiment))
+    assert not program.folded and "__shared__ _word _tb" in program.source
+
+
+def test_static_border_word_must_survive_the_fold():
+    assert _fold(StaticTwo, GOLExperiment) is None
+
+
+def test_fold_can_be_switched_off():
+    assert _fold(GameOfLife, GOLExperiment, fold=False) is None
+    program = FusedProgram(GameOfLife(GOLExperiment), fold=False)
+    assert not program.folded

@@ -23,9 +23,9 @@ from xentica_amd.runtime.fused import Batch, FusedEngine
 pytestmark = pytest.mark.gpu
 
 
-def fused(exp):
-    """The experiment with the fused engine selected."""
-    return type(exp.__name__ + "Fused", (exp,), {"engine": "fused"})
+def fused(exp, engine="fused"):
+    """The experiment with a fused engine selected."""
+    return type(exp.__name__ + "Fused", (exp,), {"engine": engine})
 
 
 def run(model, steps):
@@ -46,9 +46,11 @@ def crc(model):
     (ShiftingSands, ShiftingSandsExperiment, 1000, 1639048270),
     (NoiseTV, NoiseTVExperiment, 100, 2773894957),
 ])
-def test_golden(cls, exp, steps, want):
-    model = cls(fused(exp))
+@pytest.mark.parametrize("engine", ["fused", "fused_unfolded"])
+def test_golden(cls, exp, steps, want, engine):
+    model = cls(fused(exp, engine))
     assert isinstance(model._engine, FusedEngine)
+    assert model._engine.program.folded == (engine == "fused")
     assert crc(run(model, steps)) == want
 
 

@@ -12,7 +12,11 @@ main state and plane ``p`` buffer slot ``p`` (``xentica/core/base.py:659-661``,
 arithmetic with wrap or static-border handling (``properties.py:431-494``).
 """
 
+import re
+
 __all__ = ['PlainAddressing', 'TileAddressing', ]
+
+_INT_LITERAL = re.compile(r"^\(?\s*-?\d+\s*\)?$")
 
 
 class PlainAddressing:
@@ -36,6 +40,9 @@ class PlainAddressing:
     def store_ref(self, container):
         """C lvalue the container's packed word is stored to."""
         return self.cell_ref(container)
+
+    def note_assignment(self, prop, value):
+        """Assignments are not tracked by the plain layout."""
 
     def neighbor_declaration(self, container):
         """Coordinates, border handling and the load of a neighbour word."""
@@ -84,6 +91,22 @@ class TileAddressing:
     (``phase`` is set by the translator to ``"emit"`` or ``"absorb"``):
     which buffer slots emit stores, whether emit stores the main state or
     reads neighbours' main state, and whether absorb stores buffers.
+
+    **Emit folding.**  When every field emit stores in a buffer is either
+    a copy of the cell's own main field or an integer constant (the usual
+    "every buffer carries the state" emit), buffer slot ``p`` of a cell is
+    a fixed function of its main word: ``(main & keep_p) | const_p``
+    (the pack masks each field to its width, and a field's bits survive
+    unpack and repack unchanged).  Absorb then reads a neighbour's buffer
+    as that function of the neighbour's main word in LDS, and the kernel
+    needs neither the emit pass nor the buffer planes.  ``fold`` holds
+    ``{slot: (keep, const)}`` when this applies, else ``None``; it is
+    decided by :meth:`start_absorb` after emit has been translated.
+    Slots emit never stores read as zero, the reference's zero-initialised
+    planes; under a static border the fold is used only if it maps the
+    border word to itself in every slot (an off-board neighbour reads as
+    the border word in every slot).  A buffer field emit leaves unassigned
+    holds an indeterminate value in the reference; folding gives it 0.
     """
 
     name = "tile"
@@ -96,6 +119,65 @@ class TileAddressing:
         self.emit_reads_neighbor_main = False
         self.emit_reads_neighbor_buffer = False
         self.absorb_writes_buffer = False
+        self.fold = None
+        self._assigned = {}
+        self._slot_fold = {}
+
+    def note_assignment(self, prop, value):
+        """Remember the last value assigned to a variable during emit."""
+        if self.phase == "emit":
+            self._assigned[prop.var_name] = str(value)
+
+    def _fold_of(self, container):
+        """``(keep, const)`` of a buffer word emit stores, or ``None``."""
+        main = self._model.main
+        keep = const = shift = 0
+        for name, prop in container.properties.items():
+            mask = (1 << prop.bit_width) - 1
+            rhs = self._assigned.get(prop.var_name)
+            if rhs is None:
+                pass
+            elif rhs == main[name].var_name:
+                keep |= mask << shift
+            elif _INT_LITERAL.match(rhs):
+                const |= (int(rhs.strip("() ")) & mask) << shift
+            else:
+                return None
+            shift += prop.bit_width
+        return keep, const
+
+    def start_absorb(self, fold=True):
+        """End the emit phase: decide the fold, then translate absorb."""
+        self.fold = self._decide_fold() if fold else None
+        self.phase = "absorb"
+
+    def _decide_fold(self):
+        if self.emit_reads_neighbor_buffer or self.emit_writes_main:
+            return None
+        folds = {}
+        for slot in range(1, len(self._model.buffers) + 1):
+            if slot in self.emit_slots:
+                fold = self._slot_fold.get(slot)
+                if fold is None:
+                    return None
+            else:
+                fold = (0, 0)
+            folds[slot] = fold
+        border = self._model.topology.border
+        if hasattr(border, "off_board_state"):
+            bits = self._model.main.best_type[0]
+            value = int(border.value) & ((1 << bits) - 1)
+            if any((value & keep) | const != value
+                   for keep, const in folds.values()):
+                return None
+        return folds
+
+    def _literal(self, value):
+        """C expression of a word-sized constant."""
+        if value < 1 << 64:
+            return "((_word) %dULL)" % value
+        return "((((_word) %dULL) << 64) | (_word) %dULL)" % (
+            value >> 64, value & ((1 << 64) - 1))
 
     def _offset(self, nbr_num):
         delta = self._model.topology.neighborhood.neighbor_deltas[nbr_num]
@@ -119,15 +201,20 @@ class TileAddressing:
 
     def cell_ref(self, container):
         """Load expression of a container word."""
+        index = "_li"
         if container.nbr_num >= 0:
             if self.phase == "emit":
                 if container.buf_num == 0:
                     self.emit_reads_neighbor_main = True
                 else:
                     self.emit_reads_neighbor_buffer = True
-            plane = self._plane(container.buf_num)
-            return "(%s)[_li + %s]" % (plane, self._offset(container.nbr_num))
-        return "(%s)[_li]" % self._plane(container.buf_num)
+            index = "_li + %s" % self._offset(container.nbr_num)
+        if (self.phase == "absorb" and self.fold is not None
+                and container.buf_num > 0):
+            keep, const = self.fold[container.buf_num]
+            return "((_mp[_cur][%s] & %s) | %s)" % (
+                index, self._literal(keep), self._literal(const))
+        return "(%s)[%s]" % (self._plane(container.buf_num), index)
 
     def store_ref(self, container):
         """Store target of a container word."""
@@ -139,6 +226,7 @@ class TileAddressing:
             return "(%s)[_li]" % self._plane(0, write=True)
         if self.phase == "emit":
             self.emit_slots.add(container.buf_num)
+            self._slot_fold[container.buf_num] = self._fold_of(container)
         else:
             self.absorb_writes_buffer = True
         return "(%s)[_li]" % self._plane(container.buf_num)

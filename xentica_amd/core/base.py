@@ -313,13 +313,14 @@ class Translator:
         self.kernel_bodies['render'] = body
         return _elementwise_kernel("render", self.renderer.args, body)
 
-    def translate_tile(self):
+    def translate_tile(self, fold=True):
         """
         Translate emit and absorb+color for the fused LDS-tile kernels.
 
         Returns ``(emit_body, absorb_body, addressing)``; the addressing
         object records the access pattern the kernel must honour.  The
-        plain source is left untouched.
+        plain source is left untouched.  ``fold=False`` keeps buffer reads
+        on the buffer planes even when emit could be folded.
         """
         plain = self.addressing
         tile = TileAddressing(self)
@@ -327,7 +328,7 @@ class Translator:
         try:
             tile.phase = "emit"
             emit_body = self._translate_code(self.emit)
-            tile.phase = "absorb"
+            tile.start_absorb(fold)
             absorb_body = self._translate_code(self.absorb, self.color)
             tile.phase = None
         finally:

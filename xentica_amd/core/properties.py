@@ -162,6 +162,7 @@ class Property(DeferredExpression, BscaDetectorMixin):
     def __set__(self, obj, value):
         """Declare the variable and assign ``value`` to it."""
         self.declare_once()
+        self.bsca.addressing.note_assignment(self, value)
         self.bsca.append_code("%s = %s;\n" % (self, value))
 
     def unpack_cast(self, expr):

@@ -8,6 +8,9 @@ Step engines: how a model's step is executed on the GPU.
 * ``fused`` -- emit and absorb fused in one generated kernel with the
   buffers in LDS and several steps per launch (``runtime/fused.py``);
   steps are deferred until a launch is full or the state is read.
+* ``fused_unfolded`` -- ``fused`` with buffer reads kept on LDS buffer
+  planes even when emit only copies the state (the general form of the
+  kernel; kept selectable to check the folded one against it).
 * ``auto`` -- ``fused`` when the model qualifies, else ``plain``.
 
 Engines share one interface: ``step()`` (advance one step, possibly
@@ -80,6 +83,11 @@ def _fused(model):
     return FusedEngine(model)
 
 
+def _fused_unfolded(model):
+    from xentica_amd.runtime.fused import FusedEngine
+    return FusedEngine(model, fold=False)
+
+
 def _auto(model):
     """The fused engine when the model qualifies, else the plain one."""
     from xentica_amd.runtime.fused import FusedEngine, eligible
@@ -95,6 +103,7 @@ def _auto(model):
 ENGINES = {
     'plain': PlainEngine,
     'fused': _fused,
+    'fused_unfolded': _fused_unfolded,
     'auto': _auto,
 }
 

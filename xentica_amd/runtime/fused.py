@@ -39,14 +39,15 @@ from xentica_amd.core.topology.lattice import OrthogonalLattice
 from xentica_amd.core.topology.neighborhood import OrthogonalNeighborhood
 from xentica_amd.runtime import hip
 
-__all__ = ['FusedProgram', 'FusedEngine', 'Batch', 'eligible']
+__all__ = ['FusedProgram', 'FusedEngine', 'Batch', 'eligible', 'autotune',
+           'TUNE_TILES', 'TUNE_DEPTHS']
 
 
 class _DoesNotFit(XenticaException):
     """The requested tile and depth exceed the LDS budget."""
 
 #: LDS one workgroup may use (two workgroups per CU fit in 160 KiB).
-LDS_BUDGET = 64 * 1024
+LDS_BUDGET = 80 * 1024
 BLOCK = 256
 
 _PARAM_FIELD = {'int': ('int', np.int32), 'float': ('float', np.float32),
@@ -81,11 +82,11 @@ class FusedProgram:
     CANDIDATES = (((32, 32), 4), ((64, 16), 4), ((32, 16), 4), ((32, 32), 2),
                   ((32, 16), 2), ((16, 16), 2), ((16, 16), 1), ((8, 8), 1))
 
-    def __init__(self, model, tile=None, depth=None):
+    def __init__(self, model, tile=None, depth=None, fold=True):
         if tile is None or depth is None:
             for cand_tile, cand_depth in self.CANDIDATES:
                 try:
-                    self.__init__(model, cand_tile, cand_depth)
+                    self.__init__(model, cand_tile, cand_depth, fold)
                     return
                 except _DoesNotFit:
                     continue
@@ -93,7 +94,7 @@ class FusedProgram:
         ok, why = eligible(model)
         if not ok:
             raise XenticaException("fused engine: " + why)
-        emit_body, absorb_body, access = model.translate_tile()
+        emit_body, absorb_body, access = model.translate_tile(fold)
         if access.emit_writes_main:
             raise XenticaException(
                 "fused engine: emit() stores the main state (the reference "
@@ -102,10 +103,14 @@ class FusedProgram:
             raise XenticaException(
                 "fused engine: emit reads neighbour buffers or absorb "
                 "writes buffers")
-        self.halo_per_step = 2 if access.emit_reads_neighbor_main else 1
+        # folded emit: absorb reads buffers as functions of the main words
+        self.folded = access.fold is not None
+        self.halo_per_step = (2 if access.emit_reads_neighbor_main
+                              and not self.folded else 1)
         self.depth = int(depth)
         self.tile = (int(tile[0]), int(tile[1]))
         self.k = len(model.buffers)
+        planes = 0 if self.folded else self.k
         self.word = model.main.dtype
         self.ctype = model.main.ctype
         self.params = list({p.name: p for p in model._emit_params
@@ -118,7 +123,7 @@ class FusedProgram:
         self.region = (self.tile[0] + 2 * margin, self.tile[1] + 2 * margin)
         area = self.region[0] * self.region[1]
         itemsize = np.dtype(self.word).itemsize
-        self.lds_bytes = ((2 + self.k) * area * itemsize
+        self.lds_bytes = ((2 + planes) * area * itemsize
                           + 12 * (self.tile[0] * self.tile[1] + BLOCK))
         if self.lds_bytes > LDS_BUDGET:
             raise _DoesNotFit(
@@ -159,15 +164,21 @@ class FusedProgram:
             "_tb[%d * _RA + _li] = (_word) _BVALUE;\n" % (slot - 1)
             for slot in range(1, self.k + 1))
         border = "1" if self.static else "0"
-        tb_decl = "__shared__ _word _tb[_K * _RA];\n" if self.k else \
-            "_word *_tb = 0;\n"
+        if self.folded:
+            zero_slots = tb_decl = emit_section = ""
+        else:
+            tb_decl = "__shared__ _word _tb[_K * _RA];\n" if self.k else \
+                "_word *_tb = 0;\n"
+            emit_section = _EMIT_SECTION.format(
+                static_emit=static_emit, emit_body=emit_body)
         return _KERNEL_TEMPLATE.format(
-            defines=defines, tx=tx, ty=ty, margin=margin, k=self.k,
+            defines=defines, tx=tx, ty=ty, margin=margin,
+            k=0 if self.folded else self.k,
             he=self.halo_per_step, ctype=self.ctype, static=border,
             bvalue="%dULL" % (self.border_value & 0xFFFFFFFFFFFFFFFF),
             params_struct=params_struct, param_locals=param_locals,
-            zero_slots=zero_slots, static_emit=static_emit, tb_decl=tb_decl,
-            emit_body=emit_body, absorb_body=absorb_body)
+            zero_slots=zero_slots, tb_decl=tb_decl,
+            emit_section=emit_section, absorb_body=absorb_body)
 
     @property
     def function(self):
@@ -243,24 +254,8 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
         __syncthreads();
         int _cur = 0;
         for (int _s = 0; _s < _steps; ++_s) {{
-            const int _de = _s * _HE + (_HE - 1);
             const int _da = (_s + 1) * _HE;
-            for (int _li = threadIdx.x; _li < _RA; _li += blockDim.x) {{
-                const int _lx = _li % _RW, _ly = _li / _RW;
-                const int _d = min(min(_lx, _RW - 1 - _lx), min(_ly, _RH - 1 - _ly));
-#if _STATIC
-                const int _gx = _ox - _M + _lx, _gy = _oy - _M + _ly;
-                if (_gx < 0 || _gx >= _in.w0 || _gy < 0 || _gy >= _in.w1) {{
-                    {static_emit}
-                    continue;
-                }}
-#endif
-                if (_d < _de) continue;
-                {{
-{emit_body}
-                }}
-            }}
-            __syncthreads();
+{emit_section}
             for (int _li = threadIdx.x; _li < _RA; _li += blockDim.x) {{
                 const int _lx = _li % _RW, _ly = _li / _RW;
                 const int _d = min(min(_lx, _RW - 1 - _lx), min(_ly, _RH - 1 - _ly));
@@ -296,6 +291,27 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
         __syncthreads();
     }}
 }}
+"""
+
+
+_EMIT_SECTION = r"""
+            const int _de = _s * _HE + (_HE - 1);
+            for (int _li = threadIdx.x; _li < _RA; _li += blockDim.x) {{
+                const int _lx = _li % _RW, _ly = _li / _RW;
+                const int _d = min(min(_lx, _RW - 1 - _lx), min(_ly, _RH - 1 - _ly));
+#if _STATIC
+                const int _gx = _ox - _M + _lx, _gy = _oy - _M + _ly;
+                if (_gx < 0 || _gx >= _in.w0 || _gy < 0 || _gy >= _in.w1) {{
+                    {static_emit}
+                    continue;
+                }}
+#endif
+                if (_d < _de) continue;
+                {{
+{emit_body}
+                }}
+            }}
+            __syncthreads();
 """
 
 
@@ -370,9 +386,9 @@ class FusedEngine:
 
     name = "fused"
 
-    def __init__(self, model, tile=None, depth=None):
+    def __init__(self, model, tile=None, depth=None, fold=True):
         self.model = model
-        self.program = FusedProgram(model, tile=tile, depth=depth)
+        self.program = FusedProgram(model, tile=tile, depth=depth, fold=fold)
         self._inst = None
         self._pending = 0
         self._pending_values = None
@@ -458,14 +474,16 @@ class Batch:
     :param depth: steps per launch
     """
 
-    def __init__(self, models, tile=(32, 32), depth=4, stream=None):
+    def __init__(self, models, tile=(32, 32), depth=4, stream=None,
+                 fold=True):
         models = list(models)
         if not models:
             raise XenticaException("empty batch")
         cls = type(models[0])
         if any(type(m) is not cls for m in models):
             raise XenticaException("a batch holds instances of one class")
-        self.program = FusedProgram(models[0], tile=tile, depth=depth)
+        self.program = FusedProgram(models[0], tile=tile, depth=depth,
+                                    fold=fold)
         self.models = models
         self.stream = stream or hip.Stream()
         self.inst = _Instances(self.program, models)
@@ -553,3 +571,55 @@ class Batch:
     def synchronize(self):
         """Wait for the batch's stream."""
         self.stream.synchronize()
+
+
+#: Tile shapes and temporal depths the autotuner tries (those that fit the
+#: LDS budget for the model's word size and fold).
+TUNE_TILES = ((32, 32), (64, 16), (32, 16), (64, 32), (128, 8), (16, 16))
+TUNE_DEPTHS = (1, 2, 4, 6, 8)
+
+
+def autotune(models, trial_steps=64, tiles=TUNE_TILES, depths=TUNE_DEPTHS,
+             stream=None, log=None):
+    """
+    Pick the tile shape and temporal depth of a batch by measurement.
+
+    Every ``(tile, depth)`` pair whose kernel fits the LDS budget is
+    compiled, run once to warm up and then timed with HIP events over
+    ``trial_steps`` steps (rounded up to whole launches, graph-captured
+    as in :meth:`Batch.run`) on the models' current states; the models
+    themselves are not advanced.  Returns ``(batch, timings)``: the
+    fastest :class:`Batch`, ready to run from the models' states, and
+    ``{(tile, depth): cell-updates per second}``.
+    """
+    models = list(models)
+    stream = stream or hip.Stream()
+    cells = sum(model.cells_num for model in models)
+    timings = {}
+    best = None
+    for tile in tiles:
+        for depth in depths:
+            try:
+                FusedProgram(models[0], tile=tile, depth=depth)
+            except _DoesNotFit:
+                continue
+            batch = Batch(models, tile=tile, depth=depth, stream=stream)
+            steps = -(-trial_steps // depth) * depth
+            batch.run(steps)
+            start, stop = hip.Event(), hip.Event()
+            start.record(stream)
+            batch.run(steps)
+            stop.record(stream)
+            stop.synchronize()
+            rate = cells * steps / (start.elapsed_ms(stop) / 1e3)
+            timings[(tile, depth)] = rate
+            if log:
+                log("autotune tile %s depth %d: %.4g cell-updates/s"
+                    % (tile, depth, rate))
+            if best is None or rate > best[0]:
+                best = (rate, tile, depth)
+            del batch
+    if best is None:
+        raise XenticaException("fused engine: no tile fits in LDS")
+    _, tile, depth = best
+    return Batch(models, tile=tile, depth=depth, stream=stream), timings

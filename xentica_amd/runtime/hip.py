@@ -11,6 +11,7 @@ Compiled code objects are cached on disk, keyed by a hash of the source
 and the compiler options, so a model class compiles once per machine.
 """
 import ctypes
+import gc
 import hashlib
 import os
 import threading
@@ -253,7 +254,11 @@ class Module:
         check(library().xen_module_load(code, ctypes.byref(handle)),
               "loading module")
         self.handle = handle
-        self._functions = {}
+        # kernel handles only: a Function keeps its Module alive, and a
+        # Module that held its Functions would form a reference cycle whose
+        # unload waits for the cyclic collector (which may run while a
+        # stream is being captured, where an unload is not allowed)
+        self._handles = {}
 
     @classmethod
     def from_source(cls, source, name="xentica.hip"):
@@ -262,12 +267,12 @@ class Module:
 
     def get_function(self, name):
         """Look up a kernel by its (``extern "C"``) name."""
-        if name not in self._functions:
+        if name not in self._handles:
             handle = _vp()
             check(library().xen_module_function(
                 self.handle, name.encode(), ctypes.byref(handle)))
-            self._functions[name] = Function(self, name, handle)
-        return self._functions[name]
+            self._handles[name] = handle
+        return Function(self, name, self._handles[name])
 
     def __del__(self):
         if getattr(self, "handle", None) and _LIB is not None:
@@ -372,17 +377,41 @@ class Graph:
     def __init__(self, handle):
         self.handle = handle
 
-    @staticmethod
-    def begin(stream):
-        """Start capturing launches on ``stream``."""
-        check(library().xen_graph_begin(stream.handle), "graph capture")
+    _gc_was_enabled = None
+
+    @classmethod
+    def begin(cls, stream):
+        """
+        Start capturing launches on ``stream``.
+
+        Python's cyclic garbage collector is paused until :meth:`end`: an
+        object it frees runs ``__del__`` (``hipFree``, module unload),
+        which is not allowed on a thread that is capturing.
+        """
+        cls._gc_was_enabled = gc.isenabled()
+        gc.disable()
+        try:
+            check(library().xen_graph_begin(stream.handle), "graph capture")
+        except BaseException:
+            cls._restore_gc()
+            raise
+
+    @classmethod
+    def _restore_gc(cls):
+        if cls._gc_was_enabled:
+            gc.enable()
+        cls._gc_was_enabled = None
 
     @classmethod
     def end(cls, stream):
         """Finish capturing on ``stream`` and instantiate the graph."""
         handle = _vp()
-        check(library().xen_graph_end(stream.handle, ctypes.byref(handle)),
-              "graph instantiate")
+        try:
+            check(library().xen_graph_end(stream.handle,
+                                          ctypes.byref(handle)),
+                  "graph instantiate")
+        finally:
+            cls._restore_gc()
         return cls(handle.value)
 
     def launch(self, stream):
