@@ -276,6 +276,134 @@ def energy_life(steps=5000, plain_steps=20):
     }
 
 
+def _event_ms(stream, work, repeats):
+    """Average milliseconds of ``work()`` between HIP events on ``stream``."""
+    from xentica_amd.runtime import hip
+    work()
+    start, stop = hip.Event(), hip.Event()
+    stream.synchronize()
+    start.record(stream)
+    for _ in range(repeats):
+        work()
+    stop.record(stream)
+    stop.synchronize()
+    return start.elapsed_ms(stop) / repeats
+
+
+def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
+              eval_word=199, train_steps=800, repeats=20):
+    """
+    The neural-CA surrogate on 4096x4096 sand heights (0-8, word 42
+    family): trained on exact-engine pairs (state at t, state at t + 16),
+    scored on a held-out field, and timed to the coarse field (256x256
+    block means 16 steps ahead) against the exact engine.
+    """
+    import torch
+    from examples.sand_heights import SandHeights, sand_experiment
+    from xentica_amd.runtime.fused import autotune
+    from xentica_amd.surrogate.network import (
+        NCASurrogate, NCAFunction, encode, forward_raw)
+    from xentica_amd.surrogate.train import (
+        pairs_from_model, train, evaluate, value_mask)
+    size = (4096, 4096)
+    h, w = size[1] // 16, size[0] // 16
+    data = [pairs_from_model(SandHeights(sand_experiment(word, size)), pairs)
+            for word in train_words]
+    inputs = torch.cat([d[0] for d in data])
+    targets = torch.cat([d[1] for d in data])
+    eval_in, eval_out = pairs_from_model(
+        SandHeights(sand_experiment(eval_word, size)), 2)
+    net = NCASurrogate(seed=0)
+    t0 = time.perf_counter()
+    losses = train(net, inputs, targets, steps=train_steps, batch=4)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    score = evaluate(net, eval_in, eval_out)
+
+    # time to the coarse field: exact engine (fused, autotuned, graph) for
+    # 16 steps + block means, against encoder + surrogate forward
+    model = SandHeights(sand_experiment(eval_word, size))
+    mask = value_mask(model)
+    batch, _ = autotune([model], trial_steps=16)
+    stream = batch.stream
+    x = torch.empty((1, h, w, 16), device="cuda")
+    y = torch.empty_like(x)
+    mean = torch.empty((1, h, w), device="cuda")
+    pred = torch.empty((1, h, w), device="cuda")
+    params = net.inference_params()
+    torch.cuda.synchronize()
+
+    def exact():
+        batch.run(16)
+        encode(batch.state[batch.cur].ptr, 1, h, w, x, mean, mask, stream)
+
+    def fast():
+        encode(batch.state[batch.cur].ptr, 1, h, w, x, None, mask, stream)
+        forward_raw(x, params, y, pred, stream)
+
+    exact_ms = _event_ms(stream, exact, repeats)
+    fast_ms = _event_ms(stream, fast, repeats)
+
+    # the reference-structured plain engine for the same 16 steps
+    plain = SandHeights(type("SandPlain", (sand_experiment(eval_word, size),),
+                             {"engine": "plain"}))
+    plain.step()
+    hip_sync = __import__("xentica_amd.runtime.hip", fromlist=["hip"])
+    hip_sync.device().synchronize()
+    t0 = time.perf_counter()
+    for _ in range(16):
+        plain.step()
+    hip_sync.device().synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3
+
+    # kernel throughput at a training-sized batch
+    big = inputs[:16].contiguous()
+    big_y = torch.empty_like(big)
+    cells = big.shape[0] * h * w
+    ktensors = [t.detach().clone().requires_grad_()
+                for t in net.kernel_tensors()]
+    fwd_ms = _event_ms(stream, lambda: forward_raw(big, params, big_y, None,
+                                                   stream), repeats)
+    grad_out = torch.ones_like(big)
+
+    def fwd_bwd():
+        NCAFunction.apply(big, *ktensors).backward(grad_out)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        fwd_bwd()
+    torch.cuda.synchronize()
+    fb_ms = (time.perf_counter() - t0) * 1e3 / repeats
+    mfma_flop = 2 * (16 * 64 + 64 * 16)
+    bytes_cell = 16 * 4 * 2
+    return {
+        "workload": "neural-CA surrogate on 4096x4096 sand heights 0-8: 3x3 "
+                    "depthwise (16 ch) + 1x1 MLP 16-64-16, bf16 MFMA / fp32 "
+                    "accumulate; predicts 256x256 block means 16 steps ahead",
+        "train": {"pairs": int(inputs.shape[0]), "steps": train_steps,
+                  "seconds": train_s, "loss_first": losses[0],
+                  "loss_last": losses[-1]},
+        "heldout": score,
+        "time_to_coarse_field_ms": {"surrogate": fast_ms,
+                                    "exact_fused": exact_ms,
+                                    "exact_plain": plain_ms},
+        "speedup_vs_exact": exact_ms / fast_ms,
+        "speedup_vs_plain": plain_ms / fast_ms,
+        "exact_engine": "fused, tile %s depth %d, graph" % (
+            list(batch.program.tile), batch.program.depth),
+        "forward_kernel": {
+            "cells": cells, "ms": fwd_ms,
+            "mfma_TFLOPs": mfma_flop * cells / (fwd_ms / 1e3) / 1e12,
+            "mfma_frac_of_bf16_peak":
+                mfma_flop * cells / (fwd_ms / 1e3) / 2.5e15,
+            "algorithmic_GBps": bytes_cell * cells / (fwd_ms / 1e3) / 1e9,
+            "hbm_frac": bytes_cell * cells / (fwd_ms / 1e3) / 1e9
+                        / HBM_PEAK_GBS},
+        "forward_backward_ms": fb_ms,
+    }
+
+
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -288,6 +416,7 @@ def main():
     parser.add_argument("--no-batch", action="store_true")
     parser.add_argument("--no-3d", action="store_true")
     parser.add_argument("--no-energy", action="store_true")
+    parser.add_argument("--no-surrogate", action="store_true")
     args = parser.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -360,6 +489,9 @@ def main():
     if not args.no_batch:
         log("batched workload ...")
         result["batched"] = batched()
+    if not args.no_surrogate:
+        log("surrogate ...")
+        result["surrogate"] = surrogate()
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(cells)

@@ -92,6 +92,22 @@ def library_path():
     return os.path.join(_PKG_DIR, _LIB_NAME)
 
 
+def _share_torch_runtime():
+    """
+    Load PyTorch's HIP runtime before this library, when PyTorch is there.
+
+    PyTorch ships its own ``libamdhip64.so`` under the same soname as the
+    system one; the dynamic loader keeps whichever comes first for the
+    whole process.  Loading PyTorch first gives one HIP runtime that both
+    its tensors (device memory, streams) and these kernels use; the other
+    order leaves PyTorch unable to see the GPU.
+    """
+    try:
+        import torch  # noqa: F401  pylint: disable=import-outside-toplevel
+    except ImportError:
+        pass
+
+
 def library():
     """Load ``libxentica_hip.so`` once; raise if it was never built."""
     global _LIB  # pylint: disable=global-statement
@@ -104,6 +120,7 @@ def library():
                 raise XenticaException(
                     "%s is not built: run __graft_entry__.build() "
                     "(hipcc --offload-arch=gfx950)" % path)
+            _share_torch_runtime()
             lib = ctypes.CDLL(path)
             for name, argtypes in _SIGNATURES.items():
                 func = getattr(lib, name)
