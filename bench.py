@@ -198,26 +198,26 @@ def life3d(steps=200):
     stream = hip.Stream()
     out = {"workload": "3-D Life-like B5/S45, reference 26-offset Moore list, "
                        "1024^3 torus, 10 %% density, %d steps" % steps}
-    for name, cls, bpc in (("bitpacked", Life3DField, 0.25),
-                           ("uint8", Life3DBytes, 2.0)):
-        field = cls(size, born, sustain, stream=stream)
+    legs = (("bitpacked", Life3DField, {}, 0.25, steps),
+            ("uint8", Life3DBytes, {}, 2.0, steps),
+            ("uint8_direct", Life3DBytes, {"direct": True}, 2.0, 20))
+    for name, cls, kwargs, bpc, count in legs:
+        field = cls(size, born, sustain, stream=stream, **kwargs)
         field.set_cells(cells)
         field.step(2)
         start, stop = hip.Event(), hip.Event()
         stream.synchronize()
         start.record(stream)
-        field.step(steps)
+        field.step(count)
         stop.record(stream)
         stop.synchronize()
         ms = start.elapsed_ms(stop)
-        value = n * steps / (ms / 1e3)
+        value = n * count / (ms / 1e3)
         out[name] = {"value": value, "unit": "cell-updates/s",
-                     "ms_per_step": ms / steps,
+                     "ms_per_step": ms / count, "steps": count,
                      "algorithmic_GBps": value * bpc / 1e9,
                      "roofline_frac": value * bpc / 1e9 / HBM_PEAK_GBS}
         del field
-    out["uint8_layout_roofline_frac_of_packed"] = (
-        out["bitpacked"]["value"] * 2.0 / 1e9 / HBM_PEAK_GBS)
     return out
 
 
@@ -404,6 +404,15 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
     }
 
 
+_LEGS = {
+    "life3d": life3d,
+    "energy": energy_life,
+    "batch": batched,
+    "surrogate": surrogate,
+    "plain": plain_engine,
+}
+
+
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -417,11 +426,17 @@ def main():
     parser.add_argument("--no-3d", action="store_true")
     parser.add_argument("--no-energy", action="store_true")
     parser.add_argument("--no-surrogate", action="store_true")
+    parser.add_argument("--only", choices=sorted(_LEGS),
+                        help="run one secondary workload alone (profiling)")
     args = parser.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         return 2
+    if args.only:
+        log("%s only ..." % args.only)
+        print(json.dumps({args.only: _LEGS[args.only]()}), flush=True)
+        return 0
 
     from xentica_amd.runtime import hip
     from xentica_amd.runtime.lifebits import LifeBitsField

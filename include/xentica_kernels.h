@@ -52,9 +52,16 @@ int xen_life_bits_run(void *buf_a, void *buf_b, int width_words, int height,
  * advances one step from src to dst. */
 int xen_life3d_bits_step(const void *src, void *dst, int width, int height, int depth,
                          uint32_t born, uint32_t sustain, int z_tiles, xen_stream stream);
-/* Same rule on one uint8 per cell (a direct, unpacked restatement). */
+/* Same rule on one uint8 per cell (value in bit 0; output bytes 0 or 1):
+ * the bit-sliced kernel with byte loads and stores, same width limits,
+ * field < 2 GiB of bytes. */
 int xen_life3d_u8_step(const void *src, void *dst, int width, int height, int depth,
                        uint32_t born, uint32_t sustain, xen_stream stream);
+/* Same rule on one uint8 per cell, one thread per cell reading its 26
+ * neighbours directly: an independent restatement for cross-checks, any
+ * width. */
+int xen_life3d_u8_direct_step(const void *src, void *dst, int width, int height, int depth,
+                              uint32_t born, uint32_t sustain, xen_stream stream);
 
 /* ---- layout conversion between the reference plane and packed bits ----
  * cells: the reference's main-state plane (one uint8 per cell, value 0/1,

@@ -34,11 +34,15 @@ def test_small_vs_oracle(size, rule):
     bits.set_cells(cells)
     u8 = Life3DBytes(size, *rule)
     u8.set_cells(cells)
+    direct = Life3DBytes(size, *rule, direct=True)
+    direct.set_cells(cells)
     for step in range(steps):
         bits.step()
         u8.step()
+        direct.step()
         assert bits.crc32() == want[step], "bits step %d" % (step + 1)
         assert u8.crc32() == want[step], "u8 step %d" % (step + 1)
+        assert direct.crc32() == want[step], "direct step %d" % (step + 1)
 
 
 def test_dsl_model_matches_oracle():
@@ -64,15 +68,41 @@ def test_seed_chunks_equal_pattern():
 
 
 def test_full_size_bits_vs_u8():
-    """1024^3, 10 % density, 200 steps: packed and unpacked agree every 25."""
+    """
+    1024^3, 10 % density, 200 steps: the packed kernel, the byte-I/O kernel
+    and the direct 26-load stencil agree every 25 steps.
+    """
     size = (1024, 1024, 1024)
     cells = life3d_cells(size)
-    bits = Life3DField(size, B5, S45)
-    bits.set_cells(cells)
-    u8 = Life3DBytes(size, B5, S45)
-    u8.set_cells(cells)
+    fields = [Life3DField(size, B5, S45), Life3DBytes(size, B5, S45),
+              Life3DBytes(size, B5, S45, direct=True)]
+    for field in fields:
+        field.set_cells(cells)
     del cells
     for block in range(8):
-        bits.step(25)
-        u8.step(25)
-        assert bits.crc32() == u8.crc32(), "after %d steps" % (25 * (block + 1))
+        for field in fields:
+            field.step(25)
+        crcs = [field.crc32() for field in fields]
+        if not crcs[0] == crcs[1] == crcs[2]:
+            planes = [field.get_cells().reshape(size[::-1]) for field in fields]
+            for a, b in ((0, 1), (0, 2), (1, 2)):
+                diff = np.argwhere(planes[a] != planes[b])
+                print("fields %d/%d differ in %d cells, first (z, y, x) %s"
+                      % (a, b, len(diff), diff[:4].tolist()))
+        assert crcs[0] == crcs[1] == crcs[2], "after %d steps: %s" % (
+            25 * (block + 1), crcs)
+
+
+@pytest.mark.parametrize("size", [(1024, 16, 128), (512, 24, 70)])
+def test_wide_rows_and_z_tiles_vs_oracle(size):
+    """Rows of 16-32 lane words and depth >= 64 (several z tiles)."""
+    rng = np.random.default_rng(size[2])
+    cells = (rng.random(int(np.prod(size))) < 0.15).astype(np.uint8)
+    want, _ = oracle_run(cells, size, B5, S45, 4)
+    fields = [Life3DField(size, B5, S45), Life3DBytes(size, B5, S45)]
+    for field in fields:
+        field.set_cells(cells)
+    for step in range(4):
+        for idx, field in enumerate(fields):
+            field.step()
+            assert field.crc32() == want[step], (idx, step + 1)

@@ -226,9 +226,9 @@ __device__ __forceinline__ float wave_sum16(float v) {
     return v;
 }
 
-// Backward: every wave accumulates the gradient of its chunks in registers
-// and writes one row of `partials` (kParams floats); reduce_kernel sums
-// the rows in a fixed order.
+// Backward: every wave accumulates the gradient of its chunks in registers;
+// the workgroup adds its waves in LDS and writes one row of `partials`
+// (kParams floats); reduce_kernel sums the rows in a fixed order.
 __global__ __launch_bounds__(kBlock) void backward_kernel(
         const float *__restrict__ x, const float *__restrict__ dy, long long chunks, int h,
         int w, const float *__restrict__ dw, const float *__restrict__ bdw,
@@ -330,37 +330,73 @@ __global__ __launch_bounds__(kBlock) void backward_kernel(
         wave_sync();
     }
 
-    float *row = partials + ((long long)blockIdx.x * kWaves + wave) * kParams;
+    // sums over the 16 cells of a lane group, in every wave at once
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
-        for (int k = 0; k < kTaps; ++k) {
-            const float v = wave_sum16(g_dw[i][k]);
-            if (m == 0) row[kOffDw + (4 * g + i) * kTaps + k] = v;
-        }
-        const float vb = wave_sum16(g_bdw[i]);
-        if (m == 0) row[kOffBdw + 4 * g + i] = vb;
-        const float v2 = wave_sum16(g_b2[i]);
-        if (m == 0) row[kOffB2 + 4 * g + i] = v2;
+        for (int k = 0; k < kTaps; ++k) g_dw[i][k] = wave_sum16(g_dw[i][k]);
+        g_bdw[i] = wave_sum16(g_bdw[i]);
+        g_b2[i] = wave_sum16(g_b2[i]);
 #pragma unroll
-        for (int blk = 0; blk < 4; ++blk) {
-            const float v1 = wave_sum16(g_b1[blk][i]);
-            if (m == 0) row[kOffB1 + 16 * blk + 4 * g + i] = v1;
-            // D of dW2 tile: [out 4g+i][hid 16 blk + m]; of dW1: [hid 16 blk+4g+i][in m]
-            row[kOffW2 + (4 * g + i) * kHid + 16 * blk + m] = g_w2[blk][i];
-            row[kOffW1 + (16 * blk + 4 * g + i) * kC + m] = g_w1[blk][i];
-        }
+        for (int blk = 0; blk < 4; ++blk) g_b1[blk][i] = wave_sum16(g_b1[blk][i]);
     }
+    // combine the waves of the workgroup in LDS in wave order, then write
+    // one row of partial sums per workgroup
+    __shared__ float wg_grad[kParams];
+    for (int turn = 0; turn < kWaves; ++turn) {
+        if (wave == turn) {
+            auto put = [&](int idx, float v) {
+                if (turn == 0)
+                    wg_grad[idx] = v;
+                else
+                    wg_grad[idx] += v;
+            };
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (m == 0) {
+#pragma unroll
+                    for (int k = 0; k < kTaps; ++k) put(kOffDw + (4 * g + i) * kTaps + k, g_dw[i][k]);
+                    put(kOffBdw + 4 * g + i, g_bdw[i]);
+                    put(kOffB2 + 4 * g + i, g_b2[i]);
+#pragma unroll
+                    for (int blk = 0; blk < 4; ++blk) put(kOffB1 + 16 * blk + 4 * g + i, g_b1[blk][i]);
+                }
+#pragma unroll
+                for (int blk = 0; blk < 4; ++blk) {
+                    // D of dW2 tile: [out 4g+i][hid 16 blk + m]; of dW1:
+                    // [hid 16 blk + 4g+i][in m]
+                    put(kOffW2 + (4 * g + i) * kHid + 16 * blk + m, g_w2[blk][i]);
+                    put(kOffW1 + (16 * blk + 4 * g + i) * kC + m, g_w1[blk][i]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    float *row = partials + (long long)blockIdx.x * kParams;
+    for (int q = threadIdx.x; q < kParams; q += kBlock) row[q] = wg_grad[q];
 }
+
+// grads[p] = sum of partials[r][p] over the rows: 16 parameters x 16 row
+// slices per workgroup, the slices added in a fixed tree order.
+constexpr int kRedParams = 16;
+constexpr int kRedSlices = kBlock / kRedParams;
 
 __global__ __launch_bounds__(kBlock) void reduce_kernel(const float *__restrict__ partials,
                                                         long long rows,
                                                         float *__restrict__ grads) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= kParams) return;
+    __shared__ float part[kRedSlices][kRedParams];
+    const int lp = threadIdx.x % kRedParams, slice = threadIdx.x / kRedParams;
+    const int p = blockIdx.x * kRedParams + lp;
     float s = 0.f;
-    for (long long r = 0; r < rows; ++r) s += partials[r * kParams + p];
-    grads[p] = s;
+    if (p < kParams)
+        for (long long r = slice; r < rows; r += kRedSlices) s += partials[r * kParams + p];
+    part[slice][lp] = s;
+    __syncthreads();
+    for (int half = kRedSlices / 2; half > 0; half /= 2) {
+        if (slice < half) part[slice][lp] += part[slice + half][lp];
+        __syncthreads();
+    }
+    if (slice == 0 && p < kParams) grads[p] = part[0][lp];
 }
 
 int check_grid(long long batch, int h, int w, long long *chunks) {
@@ -410,7 +446,7 @@ int xen_nca_forward(const float *x, long long batch, int h, int w, const float *
 long long xen_nca_partial_rows(long long batch, int h, int w) {
     long long chunks;
     if (check_grid(batch, h, w, &chunks)) return -1;
-    return static_cast<long long>(wave_groups(chunks, 256)) * kWaves;
+    return wave_groups(chunks, 256);
 }
 
 int xen_nca_backward(const float *x, const float *dy, long long batch, int h, int w,
@@ -420,7 +456,7 @@ int xen_nca_backward(const float *x, const float *dy, long long batch, int h, in
     long long chunks;
     if (int rc = check_grid(batch, h, w, &chunks)) return rc;
     const int groups = wave_groups(chunks, 256);
-    if (partial_rows < static_cast<long long>(groups) * kWaves)
+    if (partial_rows < groups)
         return xen::fail("xen_nca_backward: partials holds fewer rows than "
                          "xen_nca_partial_rows() asks for", 22);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -428,8 +464,8 @@ int xen_nca_backward(const float *x, const float *dy, long long batch, int h, in
                        dw, bdw, reinterpret_cast<const short *>(w1), b1,
                        reinterpret_cast<const short *>(w2), b2, partials);
     if (int rc = xen::check(hipGetLastError(), "backward_kernel launch")) return rc;
-    hipLaunchKernelGGL(reduce_kernel, dim3((kParams + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                       partials, static_cast<long long>(groups) * kWaves, grads);
+    hipLaunchKernelGGL(reduce_kernel, dim3((kParams + kRedParams - 1) / kRedParams),
+                       dim3(kBlock), 0, s, partials, static_cast<long long>(groups), grads);
     return xen::check(hipGetLastError(), "reduce_kernel launch");
 }
 

@@ -189,9 +189,17 @@ class Life3DField:
 
 
 class Life3DBytes:
-    """The same 3-D rule on one uint8 per cell (the unpacked restatement)."""
+    """
+    The same 3-D rule on one uint8 per cell.
 
-    def __init__(self, size, born, sustain, stream=None):
+    ``direct=False``: the bit-sliced kernel with byte loads and stores
+    (widths as for :class:`Life3DField`); ``direct=True``: one thread per
+    cell reading its 26 neighbours, any width (an independent check).
+    """
+
+    def __init__(self, size, born, sustain, stream=None, direct=False):
+        self._step = (kernels.life3d_u8_direct_step if direct
+                      else kernels.life3d_u8_step)
         self.size = tuple(int(v) for v in size)
         self.n_cells = self.size[0] * self.size[1] * self.size[2]
         self.born, self.sustain = int(born), int(sustain)
@@ -223,9 +231,8 @@ class Life3DBytes:
     def step(self, steps=1):
         """Advance ``steps`` steps (asynchronous)."""
         for _ in range(steps):
-            kernels.life3d_u8_step(self.state_ptr, self._bufs[1 - self._cur].ptr,
-                                   self.size, self.born, self.sustain,
-                                   self.stream)
+            self._step(self.state_ptr, self._bufs[1 - self._cur].ptr,
+                       self.size, self.born, self.sustain, self.stream)
             self._cur = 1 - self._cur
 
     def synchronize(self):
