@@ -83,3 +83,18 @@ def test_fold_can_be_switched_off():
     assert _fold(GameOfLife, GOLExperiment, fold=False) is None
     program = FusedProgram(GameOfLife(GOLExperiment), fold=False)
     assert not program.folded
+
+
+def test_main_copy_dropped_when_absorb_stores_main():
+    program = FusedProgram(GameOfLife(GOLExperiment))
+    assert not program.main_copy
+    assert "_mp[_cur ^ 1][_li] = _mp[_cur][_li];" not in program.source
+
+
+def test_color_guard_only_for_pure_colors():
+    from examples.noisetv import NoiseTV, NoiseTVExperiment
+    life = FusedProgram(GameOfLife(GOLExperiment))
+    assert life.color_guard and "if (_d >= _M) {" in life.source
+    # NoiseTV's colour advances the generator: it must run on every cell
+    noise = FusedProgram(NoiseTV(NoiseTVExperiment))
+    assert not noise.color_guard and "if (_d >= _M) {" not in noise.source

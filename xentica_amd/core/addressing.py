@@ -119,12 +119,17 @@ class TileAddressing:
         self.emit_reads_neighbor_main = False
         self.emit_reads_neighbor_buffer = False
         self.absorb_writes_buffer = False
+        self.absorb_writes_main = False
         self.fold = None
+        self.absorb_code = self.color_code = self.store_code = ""
+        self.color_is_pure = False
+        self.assign_count = 0
         self._assigned = {}
         self._slot_fold = {}
 
     def note_assignment(self, prop, value):
-        """Remember the last value assigned to a variable during emit."""
+        """Count assignments; remember the last value of each during emit."""
+        self.assign_count += 1
         if self.phase == "emit":
             self._assigned[prop.var_name] = str(value)
 
@@ -223,6 +228,8 @@ class TileAddressing:
         if container.buf_num == 0:
             if self.phase == "emit":
                 self.emit_writes_main = True
+            elif self.phase == "absorb":
+                self.absorb_writes_main = True
             return "(%s)[_li]" % self._plane(0, write=True)
         if self.phase == "emit":
             self.emit_slots.add(container.buf_num)

@@ -188,6 +188,9 @@ class _IdentitySet:
     def __contains__(self, obj):
         return id(obj) in self._items
 
+    def __len__(self):
+        return len(self._items)
+
 
 class Translator:
     """Python -> C translation by execution (``base.py:239-524``)."""
@@ -206,17 +209,33 @@ class Translator:
         self.addressing = PlainAddressing(self)
         self.kernel_bodies = {}
 
-    def _translate_code(self, *funcs):
-        """Run the DSL methods and return the generated body."""
+    def _translate_code(self, *funcs, marks=None):
+        """
+        Run the DSL methods and return the generated body.
+
+        ``marks``, if a list, receives one ``(code pieces, deferred writes,
+        declarations, unpacks, property assignments)`` count tuple before
+        each method and one at the end of the methods (before the deferred
+        stores).
+        """
         self._func_body = []
         self._deferred_writes = {}
         self._declarations = _IdentitySet()
         self._unpacks = _IdentitySet()
         self._params = {}
         self._coords_declared = False
+
+        def mark():
+            if marks is not None:
+                marks.append((len(self._func_body), len(self._deferred_writes),
+                              len(self._declarations), len(self._unpacks),
+                              getattr(self.addressing, "assign_count", 0)))
+
         with _context.translating(self):
             for func in funcs:
+                mark()
                 func()
+            mark()
             # stores happen at the end of the body, in first-write order
             for prop in list(self._deferred_writes.values()):
                 prop.deferred_write()
@@ -325,15 +344,33 @@ class Translator:
         plain = self.addressing
         tile = TileAddressing(self)
         self.addressing = tile
+        marks = []
         try:
             tile.phase = "emit"
             emit_body = self._translate_code(self.emit)
             tile.start_absorb(fold)
-            absorb_body = self._translate_code(self.absorb, self.color)
+            self._translate_code(self.absorb, self.color, marks=marks)
+            pieces = list(self._func_body)
             tile.phase = None
         finally:
             self.addressing = plain
-        return emit_body, absorb_body, tile
+        (c_code, c_defer, c_decl, c_unp, c_asg), \
+            (e_code, e_defer, e_decl, e_unp, e_asg) = marks[1:]
+        tile.absorb_code = "".join(pieces[:c_code])
+        tile.color_code = "".join(pieces[c_code:e_code])
+        tile.store_code = "".join(pieces[e_code:])
+        # the colour may be skipped where it is discarded only if it leaves
+        # the state alone: no assignment to a field (a generator advance is
+        # one), no new stores, no first unpack or declaration of a field
+        # that the stores after it would then use
+        tile.color_is_pure = (e_asg == c_asg and e_defer == c_defer
+                              and e_unp == c_unp
+                              and not any(
+                                  isinstance(obj, Property)
+                                  for obj in list(
+                                      self._declarations._items.values()
+                                  )[c_decl:e_decl]))
+        return emit_body, "".join(pieces), tile
 
     def index_to_coord(self, i):
         """Coordinates of flat index ``i`` (or an index array)."""

@@ -133,6 +133,14 @@ class FusedProgram:
         defines = "".join(
             const.get_define_code() for name, const in model.constants.items()
             if not name.startswith(model.topology.lattice.width_prefix))
+        # the main word of a computed cell is copied forward unless absorb
+        # always stores it; the colour of halo cells (never written back) is
+        # skipped when the colour code leaves the state alone
+        self.main_copy = not access.absorb_writes_main
+        self.color_guard = access.color_is_pure
+        if self.color_guard:
+            absorb_body = (access.absorb_code + "if (_d >= _M) {\n"
+                           + access.color_code + "}\n" + access.store_code)
         self.source = self._render(defines, emit_body, absorb_body, margin)
         self._module = None
 
@@ -178,7 +186,9 @@ class FusedProgram:
             bvalue="%dULL" % (self.border_value & 0xFFFFFFFFFFFFFFFF),
             params_struct=params_struct, param_locals=param_locals,
             zero_slots=zero_slots, tb_decl=tb_decl,
-            emit_section=emit_section, absorb_body=absorb_body)
+            emit_section=emit_section, absorb_body=absorb_body,
+            main_copy=("                _mp[_cur ^ 1][_li] = _mp[_cur][_li];"
+                       if self.main_copy else ""))
 
     @property
     def function(self):
@@ -215,7 +225,10 @@ int tiles_x;
 int tile0;
 {params_struct}}};
 
+// torus wrap; only coordinates outside [0, n) (tiles on a field's edge)
+// pay for the division
 __device__ __forceinline__ int _wrap(int v, int n) {{
+    if ((unsigned) v < (unsigned) n) return v;
     v %= n;
     return v < 0 ? v + n : v;
 }}
@@ -267,7 +280,7 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
                 }}
 #endif
                 if (_d < _da) continue;
-                _mp[_cur ^ 1][_li] = _mp[_cur][_li];
+{main_copy}
                 const int _oi = _d >= _M ? (_ly - _M) * _TX + (_lx - _M)
                                          : _TX * _TY + (int) threadIdx.x;
                 {{
