@@ -17,27 +17,44 @@ def wrap(v, n):
     v = v % n
     return v
 
-def life_word(u, m, d, born, sustain, fast):
-    b0 = u ^ d; b1 = u & d; a0 = b0 ^ m; a1 = (m & b0) | b1
-    # DPP wave_shr:1 (lane i gets i-1), bound_ctrl -> 0 for lane 0
-    a0l = np.concatenate([[0], a0[:-1]]).astype(np.uint64); a1l = np.concatenate([[0], a1[:-1]]).astype(np.uint64)
-    a0r = np.concatenate([a0[1:], [0]]).astype(np.uint64); a1r = np.concatenate([a1[1:], [0]]).astype(np.uint64)
-    l0 = ((a0 << 1) | (a0l >> 31)) & M32; l1 = ((a1 << 1) | (a1l >> 31)) & M32
-    r0 = ((a0 >> 1) | (a0r << 31)) & M32; r1 = ((a1 >> 1) | (a1r << 31)) & M32
-    c0 = l0 ^ r0 ^ b0; k0 = (l0 & r0) | (b0 & (l0 | r0))
-    x = l1 ^ r1 ^ b1; y = (l1 & r1) | (b1 & (l1 | r1))
+def maj(a, b, c):
+    return (a & b) | (c & (a | b))
+
+def hrow(w):
+    """A row word with its horizontal 3-sums (h0 + 2 h1 = left + self + right)."""
+    # DPP wave_shr:1 / wave_shl:1 with bound_ctrl: zero at the wave edges
+    wl = np.concatenate([[0], w[:-1]]).astype(np.uint64)
+    wr = np.concatenate([w[1:], [0]]).astype(np.uint64)
+    l = ((w << 1) | (wl >> 31)) & M32
+    r = ((w >> 1) | (wr << 31)) & M32
+    return (w, l ^ w ^ r, maj(l, w, r))
+
+def life_word(a, b, c, born, sustain, fast):
+    """New middle row from the (raw, h0, h1) of rows up, mid, down."""
+    t0 = a[1] ^ b[1] ^ c[1]
+    k = maj(a[1], b[1], c[1])
+    u = a[2] ^ b[2] ^ c[2]
+    v = maj(a[2], b[2], c[2])
+    alive = b[0]
+    # count9 (3x3 block with the cell) = t0 + 2 (u + k + 2 v)
     if fast:
-        t = (x ^ k0) & (~y & M32)
-        return t & (c0 | m)
-    c1 = x ^ k0; k1 = x & k0; c2 = y ^ k1; c3 = y & k1
-    res = np.zeros_like(m)
-    for v in range(9):
-        eq = np.full_like(m, M32)
-        for bit, c in enumerate((c0, c1, c2, c3)):
-            eq &= c if (v >> bit) & 1 else (~c & M32)
-        bm = M32 if (born >> v) & 1 else np.uint64(0)
-        sm = M32 if (sustain >> v) & 1 else np.uint64(0)
-        res |= eq & ((m & (sm | bm)) | (~m & M32 & bm))
+        e1 = (u ^ k) & (~v & M32)                                # W == 1
+        e2 = (u & k & (~v & M32)) | ((~u & M32) & (~k & M32) & v)  # W == 2
+        f = e2 & alive & (~t0 & M32)
+        return (t0 & e1) | f
+    w0 = u ^ k
+    w1 = v ^ (u & k)
+    w2 = v & u & k
+    res = np.zeros_like(alive)
+    for c9 in range(10):
+        eq = np.full_like(alive, M32)
+        for bit, cb in enumerate((t0, w0, w1, w2)):
+            eq &= cb if (c9 >> bit) & 1 else (~cb & M32)
+        dead_next = M32 if c9 <= 8 and (born >> c9) & 1 else np.uint64(0)
+        n = c9 - 1
+        live_next = M32 if 0 <= n <= 8 and ((born | sustain) >> n) & 1 \
+            else np.uint64(0)
+        res |= eq & ((alive & live_next) | ((~alive & M32) & dead_next))
     return res
 
 def kernel(src, W, H, tiles_x, tiles_y, K, born, sustain):
@@ -72,11 +89,12 @@ def kernel(src, W, H, tiles_x, tiles_y, K, born, sustain):
                     dst[off // 4] = val[ln]
         lev = [[None] * 3 for _ in range(K + 1)]
         def feed(i, inp, all_levels):
-            lev[0][i % 3] = inp
+            lev[0][i % 3] = hrow(inp)
             for l in range(1, K + 1):
                 if all_levels or i >= 2 * l:
                     j3 = (i - l) % 3
-                    lev[l][j3] = life_word(lev[l-1][(j3 + 2) % 3], lev[l-1][j3], lev[l-1][(j3 + 1) % 3], born, sustain, fast)
+                    v = life_word(lev[l-1][(j3 + 2) % 3], lev[l-1][j3], lev[l-1][(j3 + 1) % 3], born, sustain, fast)
+                    lev[l][j3] = hrow(v) if l < K else (v, None, None)
         for i in range(2 * K):
             feed(i, load_row(i), False)
         i = 2 * K
@@ -86,14 +104,14 @@ def kernel(src, W, H, tiles_x, tiles_y, K, born, sustain):
             na, nb, nc = load_row(i + 3), load_row(i + 4), load_row(i + 5)
             for q, v in enumerate((a, b, c)):
                 feed(i + q, v, True)
-                store_row(lev[K][(i + q - K) % 3], i + q - 2 * K)
+                store_row(lev[K][(i + q - K) % 3][0], i + q - 2 * K)
             a, b, c = na, nb, nc
             i += 3; iters += 1
             assert iters < 100000
         if i < L:
-            feed(i, a, True); store_row(lev[K][(i - K) % 3], i - 2 * K)
+            feed(i, a, True); store_row(lev[K][(i - K) % 3][0], i - 2 * K)
             if i + 1 < L:
-                feed(i + 1, b, True); store_row(lev[K][(i + 1 - K) % 3], i + 1 - 2 * K)
+                feed(i + 1, b, True); store_row(lev[K][(i + 1 - K) % 3][0], i + 1 - 2 * K)
     assert not np.any(dst == 0xDEADBEEF), "unwritten words"
     return dst
 

@@ -59,77 +59,88 @@ __device__ __forceinline__ uint32_t bitop3(uint32_t a, uint32_t b, uint32_t c) {
 constexpr uint8_t kXor3 = bitop3_table([](bool a, bool b, bool c) { return a ^ b ^ c; });
 constexpr uint8_t kMaj3 = bitop3_table([](bool a, bool b, bool c) { return (a + b + c) >= 2; });
 constexpr uint8_t kMux = bitop3_table([](bool a, bool b, bool c) { return a ? b : c; });
-constexpr uint8_t kXorAndNot = bitop3_table([](bool a, bool b, bool c) { return (a ^ b) && !c; });
-constexpr uint8_t kAndOr = bitop3_table([](bool a, bool b, bool c) { return a && (b || c); });
+// count9 = t0 + 2W, W = u + k + 2v: e1 = (W == 1), e2 = (W == 2)
+constexpr uint8_t kW1 = bitop3_table([](bool u, bool k, bool v) { return (u != k) && !v; });
+constexpr uint8_t kW2 = bitop3_table(
+    [](bool u, bool k, bool v) { return (u && k && !v) || (!u && !k && v); });
+constexpr uint8_t kE2AliveNotT0 = bitop3_table([](bool e2, bool m, bool t0) { return e2 && m && !t0; });
+constexpr uint8_t kAndOr2 = bitop3_table([](bool t0, bool e1, bool f) { return (t0 && e1) || f; });
+constexpr uint8_t kXorAnd = bitop3_table([](bool v, bool u, bool k) { return v != (u && k); });
+constexpr uint8_t kAnd3 = bitop3_table([](bool v, bool u, bool k) { return v && u && k; });
 
-// Rule application.  The neighbour count is count = c0 + 2 * s2 with
-// s2 = x + k0 + 2y (x, k0, y single bits); kFast selects B3/S23, where
-// next = (s2 == 1) & (c0 | m) = ((x ^ k0) & ~y) & (c0 | m).  Otherwise the
-// rule masks are runtime values and the lookup is a mux tree over the
-// count bits with the state muxed in at the leaves.
+// A row word with its horizontal 3-sums: h0 + 2 h1 = (x-1) + x + (x+1).
+struct HRow {
+    uint32_t raw, h0, h1;
+};
+
+__device__ __forceinline__ HRow hrow(uint32_t w) {
+    const uint32_t wl = from_left_lane(w), wr = from_right_lane(w);
+    const uint32_t l = (w << 1) | (wl >> 31);  // cells at x - 1
+    const uint32_t r = (w >> 1) | (wr << 31);  // cells at x + 1
+    return {w, bitop3<kXor3>(l, w, r), bitop3<kMaj3>(l, w, r)};
+}
+
+// Rule application on the 3x3 block sum (the cell included):
+// count9 = t0 + 2 (u + k + 2 v) from the three rows' horizontal sums, and
+// the neighbour count is count9 - state.  kFast selects B3/S23, where
+// next = (count9 == 3) | (state & count9 == 4).  Otherwise the rule masks
+// are runtime values and the lookup is a mux tree over the bits of count9
+// with the state muxed in at the leaves.
 template <bool kFast>
 struct Rule;
 
 template <>
 struct Rule<true> {
     uint32_t born, sustain;
-    __device__ __forceinline__ uint32_t apply(uint32_t c0, uint32_t x, uint32_t k0, uint32_t y,
+    __device__ __forceinline__ uint32_t apply(uint32_t t0, uint32_t u, uint32_t k, uint32_t v,
                                               uint32_t m) const {
-        return bitop3<kAndOr>(bitop3<kXorAndNot>(x, k0, y), c0, m);
+        const uint32_t e1 = bitop3<kW1>(u, k, v);
+        const uint32_t e2 = bitop3<kW2>(u, k, v);
+        return bitop3<kAndOr2>(t0, e1, bitop3<kE2AliveNotT0>(e2, m, t0));
     }
 };
 
 template <>
 struct Rule<false> {
     uint32_t born, sustain;
-    __device__ __forceinline__ uint32_t apply(uint32_t c0, uint32_t x, uint32_t k0, uint32_t y,
+    __device__ __forceinline__ uint32_t apply(uint32_t t0, uint32_t u, uint32_t k, uint32_t v,
                                               uint32_t m) const {
-        const uint32_t c1 = x ^ k0;
-        const uint32_t k1 = x & k0;
-        const uint32_t c2 = y ^ k1;
-        const uint32_t c3 = y & k1;
-        // leaf[v]: next state for count v; a live cell is kept when v is
-        // a birth OR a survival count (next = born | sustained & state)
-        uint32_t leaf[9];
+        const uint32_t w0 = u ^ k;
+        const uint32_t w1 = bitop3<kXorAnd>(v, u, k);
+        const uint32_t w2 = bitop3<kAnd3>(v, u, k);
+        // leaf[c]: next state for count9 = c; a dead cell has c neighbours,
+        // a live one c - 1 and is kept when that is a birth OR a survival
+        // count (next = born | sustained & state)
+        uint32_t leaf[10];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) {
-            const uint32_t b = ((born >> v) & 1u) ? 0xFFFFFFFFu : 0u;
-            const uint32_t s = ((sustain >> v) & 1u) ? 0xFFFFFFFFu : 0u;
-            leaf[v] = bitop3<kMux>(m, s | b, b);
+        for (int c = 0; c < 10; ++c) {
+            const uint32_t d = (c <= 8 && ((born >> c) & 1u)) ? 0xFFFFFFFFu : 0u;
+            const uint32_t a =
+                (c >= 1 && (((born | sustain) >> (c - 1)) & 1u)) ? 0xFFFFFFFFu : 0u;
+            leaf[c] = bitop3<kMux>(m, a, d);
         }
-        const uint32_t l0 = bitop3<kMux>(c0, leaf[1], leaf[0]);
-        const uint32_t l1 = bitop3<kMux>(c0, leaf[3], leaf[2]);
-        const uint32_t l2 = bitop3<kMux>(c0, leaf[5], leaf[4]);
-        const uint32_t l3 = bitop3<kMux>(c0, leaf[7], leaf[6]);
-        const uint32_t m0 = bitop3<kMux>(c1, l1, l0);
-        const uint32_t m1 = bitop3<kMux>(c1, l3, l2);
-        const uint32_t n0 = bitop3<kMux>(c2, m1, m0);
-        return bitop3<kMux>(c3, leaf[8], n0);
+        const uint32_t l0 = bitop3<kMux>(t0, leaf[1], leaf[0]);
+        const uint32_t l1 = bitop3<kMux>(t0, leaf[3], leaf[2]);
+        const uint32_t l2 = bitop3<kMux>(t0, leaf[5], leaf[4]);
+        const uint32_t l3 = bitop3<kMux>(t0, leaf[7], leaf[6]);
+        const uint32_t l4 = bitop3<kMux>(t0, leaf[9], leaf[8]);
+        const uint32_t m0 = bitop3<kMux>(w0, l1, l0);
+        const uint32_t m1 = bitop3<kMux>(w0, l3, l2);
+        const uint32_t n0 = bitop3<kMux>(w1, m1, m0);
+        return bitop3<kMux>(w2, l4, n0);
     }
 };
 
-// New state of one word (32 cells) from its column's up/mid/down words.
+// New state of one word (32 cells) from the horizontal sums of its rows
+// up (a), mid (b) and down (c).
 template <bool kFast>
-__device__ __forceinline__ uint32_t life_word(uint32_t u, uint32_t m, uint32_t d,
+__device__ __forceinline__ uint32_t life_word(const HRow &a, const HRow &b, const HRow &c,
                                               const Rule<kFast> &rule) {
-    // vertical sums: b = up + down (the cell's own column, without the
-    // cell), a = up + mid + down (the side columns); two bits each
-    const uint32_t b0 = u ^ d;
-    const uint32_t b1 = u & d;
-    const uint32_t a0 = b0 ^ m;
-    const uint32_t a1 = (m & b0) | b1;
-    // column sums of the left and right neighbour columns
-    const uint32_t a0l = from_left_lane(a0), a1l = from_left_lane(a1);
-    const uint32_t a0r = from_right_lane(a0), a1r = from_right_lane(a1);
-    const uint32_t l0 = (a0 << 1) | (a0l >> 31), l1 = (a1 << 1) | (a1l >> 31);
-    const uint32_t r0 = (a0 >> 1) | (a0r << 31), r1 = (a1 >> 1) | (a1r << 31);
-    // count = (l0 + 2 l1) + (r0 + 2 r1) + (b0 + 2 b1), bit-sliced:
-    // c0 + 2 k0 = l0 + r0 + b0 and x + 2 y = l1 + r1 + b1
-    const uint32_t c0 = bitop3<kXor3>(l0, r0, b0);
-    const uint32_t k0 = bitop3<kMaj3>(l0, r0, b0);
-    const uint32_t x = bitop3<kXor3>(l1, r1, b1);
-    const uint32_t y = bitop3<kMaj3>(l1, r1, b1);
-    return rule.apply(c0, x, k0, y, m);
+    const uint32_t t0 = bitop3<kXor3>(a.h0, b.h0, c.h0);
+    const uint32_t k = bitop3<kMaj3>(a.h0, b.h0, c.h0);
+    const uint32_t u = bitop3<kXor3>(a.h1, b.h1, c.h1);
+    const uint32_t v = bitop3<kMaj3>(a.h1, b.h1, c.h1);
+    return rule.apply(t0, u, k, v, b.raw);
 }
 
 __device__ __forceinline__ int wrap(int v, int n) {
@@ -137,10 +148,11 @@ __device__ __forceinline__ int wrap(int v, int n) {
     return v < 0 ? v + n : v;
 }
 
-// Pipeline state: lev[l][j % 3] is row j of step level l (0 = input).
+// Pipeline state: lev[l][j % 3] is row j of step level l (0 = input),
+// with its horizontal sums for the levels that feed another level.
 template <int K>
 struct Window {
-    uint32_t lev[K + 1][3];
+    HRow lev[K + 1][3];
 };
 
 // Feed input row i (with i % 3 == I3) into the pipeline and advance every
@@ -149,20 +161,24 @@ struct Window {
 template <int K, bool kFast, int I3, bool kAll>
 __device__ __forceinline__ void feed(Window<K> &w, uint32_t input, int i,
                                      const Rule<kFast> &rule) {
-    w.lev[0][I3] = input;
+    w.lev[0][I3] = hrow(input);
 #pragma unroll
     for (int l = 1; l <= K; ++l) {
         if (kAll || i >= 2 * l) {
             const int j3 = ((I3 - l) % 3 + 3) % 3;
-            w.lev[l][j3] = life_word<kFast>(w.lev[l - 1][(j3 + 2) % 3], w.lev[l - 1][j3],
-                                            w.lev[l - 1][(j3 + 1) % 3], rule);
+            const uint32_t v = life_word<kFast>(w.lev[l - 1][(j3 + 2) % 3], w.lev[l - 1][j3],
+                                                w.lev[l - 1][(j3 + 1) % 3], rule);
+            if (l < K)
+                w.lev[l][j3] = hrow(v);
+            else
+                w.lev[l][j3].raw = v;  // the output level needs no sums
         }
     }
 }
 
 template <int K, int I3>
 __device__ __forceinline__ uint32_t level_k_row(const Window<K> &w) {
-    return w.lev[K][((I3 - K) % 3 + 3) % 3];
+    return w.lev[K][((I3 - K) % 3 + 3) % 3].raw;
 }
 
 // One launch = K steps over the whole field.  Wave w owns tile (tx, ty):
