@@ -67,16 +67,26 @@ def timed(field, stream, steps):
 
 
 def autotune(field, stream, trial_steps=96):
-    """Pick the temporal depth with the best throughput on this device."""
+    """
+    Pick the temporal depth and the number of row strips (as waves per
+    SIMD of the device: taller strips repeat fewer halo rows, more strips
+    hide more latency) with the best throughput on this device.
+    """
+    from xentica_amd.runtime import hip
+    simds = 4 * hip.device().compute_units
+    tiles_x = -(-(field.width // 32) // 62)
     best = None
-    for depth in (2, 3, 4, 5, 6, 8):
-        field.depth = depth
-        timed(field, stream, 2 * depth)
-        ms = timed(field, stream, trial_steps)
-        log("autotune depth=%d: %.3f ms / %d steps" % (depth, ms, trial_steps))
-        if best is None or ms < best[1]:
-            best = (depth, ms)
-    field.depth = best[0]
+    for depth in (5, 6, 8):
+        for waves in (2, 4, 6, 8):
+            strips = max(1, min(field.height, -(-waves * simds // tiles_x)))
+            field.depth, field.strips = depth, strips
+            timed(field, stream, 2 * depth)
+            ms = timed(field, stream, trial_steps)
+            log("autotune depth=%d strips=%d: %.3f ms / %d steps"
+                % (depth, strips, ms, trial_steps))
+            if best is None or ms < best[2]:
+                best = (depth, strips, ms)
+    field.depth, field.strips = best[0], best[1]
     return best[0]
 
 
@@ -479,6 +489,7 @@ def main():
             "workload": "life_b3s23_torus_16384x16384_bitpacked",
             "size": list(SIZE), "rule": "B3/S23", "seed_word": WORD,
             "density": 0.5, "temporal_depth": depth,
+            "row_strips": field.strips,
             "layout": "64 cells per uint64 (32 per uint32 lane word)",
             "device": device.name,
         },

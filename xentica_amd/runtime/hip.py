@@ -14,6 +14,7 @@ import ctypes
 import gc
 import hashlib
 import os
+import re
 import threading
 
 import numpy as np
@@ -156,6 +157,9 @@ class Device:
         buf = ctypes.create_string_buffer(256)
         check(lib.xen_device_name(buf, 256))
         self.name = buf.value.decode()
+        # xen_device_name ends with "(<arch>, <n> CUs)"
+        found = re.search(r"(\d+) CUs\)$", self.name)
+        self.compute_units = int(found.group(1)) if found else 256
 
     def synchronize(self):
         """Wait for all work on the device."""
