@@ -94,7 +94,7 @@ def test_main_copy_dropped_when_absorb_stores_main():
 def test_color_guard_only_for_pure_colors():
     from examples.noisetv import NoiseTV, NoiseTVExperiment
     life = FusedProgram(GameOfLife(GOLExperiment))
-    assert life.color_guard and "if (_d >= _M) {" in life.source
+    assert life.color_guard and "if (_inner) {" in life.source
     # NoiseTV's colour advances the generator: it must run on every cell
     noise = FusedProgram(NoiseTV(NoiseTVExperiment))
-    assert not noise.color_guard and "if (_d >= _M) {" not in noise.source
+    assert not noise.color_guard and "if (_inner) {" not in noise.source

@@ -123,6 +123,9 @@ class FusedProgram:
         self.region = (self.tile[0] + 2 * margin, self.tile[1] + 2 * margin)
         area = self.region[0] * self.region[1]
         itemsize = np.dtype(self.word).itemsize
+        if area * self.region[0] >= 1 << 22:
+            raise _DoesNotFit("fused engine: region %s too large for the "
+                              "kernel's row arithmetic" % (self.region, ))
         self.lds_bytes = ((2 + planes) * area * itemsize
                           + 12 * (self.tile[0] * self.tile[1] + BLOCK))
         if self.lds_bytes > LDS_BUDGET:
@@ -139,7 +142,7 @@ class FusedProgram:
         self.main_copy = not access.absorb_writes_main
         self.color_guard = access.color_is_pure
         if self.color_guard:
-            absorb_body = (access.absorb_code + "if (_d >= _M) {\n"
+            absorb_body = (access.absorb_code + "if (_inner) {\n"
                            + access.color_code + "}\n" + access.store_code)
         self.source = self._render(defines, emit_body, absorb_body, margin)
         self._module = None
@@ -216,6 +219,20 @@ _KERNEL_TEMPLATE = r"""
 #define _BVALUE {bvalue}
 typedef {ctype} _word;
 
+// The cells of the region at distance >= margin from its edge, one per
+// thread in turn: (_lx, _ly) and the region index _li.  The row comes
+// from a float reciprocal of the rectangle width, exact while
+// _RA * _RW < 2^22 (the host checks): (q + 0.5) / w is at least 0.5 / w
+// from an integer and the product's error is below _RA * 2^-23.
+#define _RECT_FOR(margin) \
+    const int _mg = (margin), _ws = _RW - 2 * _mg, _hs = _RH - 2 * _mg; \
+    const float _iw = 1.0f / (float) _ws; \
+    for (int _q = threadIdx.x, _ry = 0, _lx = 0, _ly = 0, _li = 0; \
+         _q < _ws * _hs && (_ry = (int) (((float) _q + 0.5f) * _iw), \
+                            _lx = _mg + (_q - _ry * _ws), _ly = _mg + _ry, \
+                            _li = _ly * _RW + _lx, true); \
+         _q += blockDim.x)
+
 struct _Inst {{
 long long cells;
 long long cols;
@@ -269,9 +286,7 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
         for (int _s = 0; _s < _steps; ++_s) {{
             const int _da = (_s + 1) * _HE;
 {emit_section}
-            for (int _li = threadIdx.x; _li < _RA; _li += blockDim.x) {{
-                const int _lx = _li % _RW, _ly = _li / _RW;
-                const int _d = min(min(_lx, _RW - 1 - _lx), min(_ly, _RH - 1 - _ly));
+            {{ _RECT_FOR(_da) {{
 #if _STATIC
                 const int _gx = _ox - _M + _lx, _gy = _oy - _M + _ly;
                 if (_gx < 0 || _gx >= _in.w0 || _gy < 0 || _gy >= _in.w1) {{
@@ -279,16 +294,16 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
                     continue;
                 }}
 #endif
-                if (_d < _da) continue;
 {main_copy}
-                const int _oi = _d >= _M ? (_ly - _M) * _TX + (_lx - _M)
-                                         : _TX * _TY + (int) threadIdx.x;
+                const bool _inner = _lx >= _M && _lx < _RW - _M && _ly >= _M && _ly < _RH - _M;
+                const int _oi = _inner ? (_ly - _M) * _TX + (_lx - _M)
+                                       : _TX * _TY + (int) threadIdx.x;
                 {{
                     int3 *col = &_col[_oi];
                     const unsigned i = 0;
 {absorb_body}
                 }}
-            }}
+            }} }}
             __syncthreads();
             _cur ^= 1;
         }}
@@ -308,10 +323,8 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
 
 
 _EMIT_SECTION = r"""
-            const int _de = _s * _HE + (_HE - 1);
-            for (int _li = threadIdx.x; _li < _RA; _li += blockDim.x) {{
-                const int _lx = _li % _RW, _ly = _li / _RW;
-                const int _d = min(min(_lx, _RW - 1 - _lx), min(_ly, _RH - 1 - _ly));
+            {{ const int _de = _s * _HE + (_HE - 1);
+            _RECT_FOR(_de) {{
 #if _STATIC
                 const int _gx = _ox - _M + _lx, _gy = _oy - _M + _ly;
                 if (_gx < 0 || _gx >= _in.w0 || _gy < 0 || _gy >= _in.w1) {{
@@ -319,11 +332,10 @@ _EMIT_SECTION = r"""
                     continue;
                 }}
 #endif
-                if (_d < _de) continue;
                 {{
 {emit_body}
                 }}
-            }}
+            }} }}
             __syncthreads();
 """
 
