@@ -3,7 +3,7 @@
  *
  * No counterpart in the reference: Xentica has no learned component.
  * These kernels serve the surrogate that predicts a coarse observable of
- * an exact model 16 steps ahead (the exact engine stays the judge): a
+ * an exact model 16 steps ahead (the exact engine stays the ground truth): a
  * coarse-graining encoder from an engine's uint8 main plane, and the
  * forward and backward passes of a 3x3 depthwise perception (16 channels,
  * fp32) followed by a 1x1 MLP 16 -> 64 -> 16 on bf16 MFMA with fp32

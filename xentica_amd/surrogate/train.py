@@ -1,7 +1,7 @@
 """
 Training pairs from the exact engine, training and evaluation.
 
-The exact engine is the judge: a training pair is the coarse encoding of
+The exact engine is the ground truth: a training pair is the coarse encoding of
 a model's state at step t (16 sub-block means per 16x16 block) and at
 step t + horizon, both produced by stepping the model itself.  The
 surrogate is scored by the L1 error of its coarse observable (block mean)
