@@ -65,3 +65,22 @@ def test_full_size_energy_conserved():
         assert energy.sum() == total
         assert energy.max() <= 255
         assert 0.05 < mode.mean() < 0.95
+
+
+def test_batch_of_uneven_fields():
+    """128-bit words in one persistent launch: every instance = its oracle."""
+    from xentica_amd.runtime.fused import Batch
+    sizes = [(96, 80), (64, 200), (130, 48)]
+    exps = [type("Energy%d" % idx, (EnergyLifeSmall, ),
+                 {"size": size, "word": 42 + idx})
+            for idx, size in enumerate(sizes)]
+    models = [EnergyLife(exp) for exp in exps]
+    batch = Batch(models, tile=(16, 16), depth=2)
+    batch.run(11)
+    batch.synchronize()
+    batch.scatter()
+    for model, exp in zip(models, exps):
+        field = oracle.energy_life_field(exp.size, exp.word)
+        oracle.energy_life(field, 11)
+        assert model_cells(model).tobytes() == field.cells.tobytes(), exp.size
+        assert np.array_equal(model.gpu.arrays.colors.get(), field.col)

@@ -45,8 +45,10 @@ __device__ __forceinline__ uint32_t op3(uint32_t a, uint32_t b, uint32_t c) {
 constexpr uint8_t kXor3 = bitop3_table([](bool a, bool b, bool c) { return a ^ b ^ c; });
 constexpr uint8_t kMaj3 = bitop3_table([](bool a, bool b, bool c) { return (a + b + c) >= 2; });
 constexpr uint8_t kMux = bitop3_table([](bool a, bool b, bool c) { return a ? b : c; });
-constexpr uint8_t kNotAAndBAndNotC = bitop3_table([](bool a, bool b, bool c) { return !a && b && !c; });
-constexpr uint8_t kAAndNotBAndC = bitop3_table([](bool a, bool b, bool c) { return a && !b && c; });
+constexpr uint8_t kNotANotBC = bitop3_table([](bool a, bool b, bool c) { return !a && !b && c; });
+constexpr uint8_t kNotAAndBOrC = bitop3_table([](bool a, bool b, bool c) { return !a && (b || c); });
+constexpr uint8_t kTwoOrOneAlive =
+    bitop3_table([](bool b1, bool b0, bool s) { return (b1 && !b0) || (!b1 && b0 && s); });
 
 __device__ __forceinline__ int wrap(int v, int n) {
     v %= n;
@@ -118,16 +120,24 @@ struct Rule3;
 template <>
 struct Rule3<true> {  // B5/S45
     uint32_t born, sustain;
-    __device__ __forceinline__ uint32_t apply(const Count5 &c, uint32_t s) const {
-        // (c == 5) | (c == 4) & s  =  ~c1 & c2 & ~c3 & ~c4 & (c0 | s)
-        return op3<kAAndNotBAndC>(op3<kNotAAndBAndNotC>(c.b1, c.b2, c.b3), c.b4, c.b0 | s);
+    // full: the 27-cell sum S; d: the dropped (1,1,1) cell; s: the state.
+    // count = S - d, next = (count == 5) | (count == 4) & s: S is 4..6, so
+    // next = (S4 == 0 & S3 == 0 & S2 == 1) & (d ? (S10 == 2 | S10 == 1 & s)
+    //                                          : (S10 == 1 | S10 == 0 & s)),
+    // with S10 the low two bits -- five logic ops, no subtraction.
+    __device__ __forceinline__ uint32_t apply(const Count5 &full, uint32_t d, uint32_t s) const {
+        const uint32_t p = op3<kNotANotBC>(full.b4, full.b3, full.b2);
+        const uint32_t g0 = op3<kNotAAndBOrC>(full.b1, full.b0, s);
+        const uint32_t g1 = op3<kTwoOrOneAlive>(full.b1, full.b0, s);
+        return p & op3<kMux>(d, g1, g0);
     }
 };
 
 template <>
 struct Rule3<false> {
     uint32_t born, sustain;
-    __device__ __forceinline__ uint32_t apply(const Count5 &c, uint32_t s) const {
+    __device__ __forceinline__ uint32_t apply(const Count5 &full, uint32_t d, uint32_t s) const {
+        const Count5 c = sub_bit(full, d);
         uint32_t leaf[32];
 #pragma unroll
         for (int v = 0; v < 32; ++v) {
@@ -252,7 +262,7 @@ __device__ __forceinline__ void emit_plane(const Plane<YS> &a, const Plane<YS> &
 #pragma unroll
     for (int y = 0; y < YS; ++y) {
         const Count5 full = add3_4(a.xy[y], b.xy[y], c.xy[y]);
-        const uint32_t next = rule.apply(sub_bit(full, c.right[y]), b.raw[y]);
+        const uint32_t next = rule.apply(full, c.right[y], b.raw[y]);
         store_word<kBytes>(next, out, out_off[y], soff);
     }
 }
