@@ -25,7 +25,8 @@ def model_cells(model):
     return model.gpu.arrays.cells.get()[:model.cells_num]
 
 
-@pytest.mark.parametrize("engine", ["plain", "fused", "fused_unfolded"])
+@pytest.mark.parametrize("engine", ["plain", "fused", "fused_unfolded",
+                                    "fused_generic"])
 def test_small_per_step(engine):
     model = EnergyLife(experiment(EnergyLifeSmall, engine))
     field = oracle.energy_life_field(EnergyLifeSmall.size)

@@ -18,7 +18,7 @@ from examples.evolife import (
     EvoLife, CrossbreedingExperiment, CrossbreedingExperiment2)
 from examples.workloads import (
     Life512Experiment, LifeLikeRule, batch_experiments)
-from xentica_amd.runtime.fused import Batch, FusedEngine
+from xentica_amd.runtime.fused import Batch, FusedEngine, TUNE_TILES
 
 pytestmark = pytest.mark.gpu
 
@@ -46,11 +46,13 @@ def crc(model):
     (ShiftingSands, ShiftingSandsExperiment, 1000, 1639048270),
     (NoiseTV, NoiseTVExperiment, 100, 2773894957),
 ])
-@pytest.mark.parametrize("engine", ["fused", "fused_unfolded"])
+@pytest.mark.parametrize("engine", ["fused", "fused_unfolded",
+                                    "fused_generic"])
 def test_golden(cls, exp, steps, want, engine):
     model = cls(fused(exp, engine))
     assert isinstance(model._engine, FusedEngine)
-    assert model._engine.program.folded == (engine == "fused")
+    assert model._engine.program.folded == (engine != "fused_unfolded")
+    assert model._engine.program.split == (engine == "fused")
     assert crc(run(model, steps)) == want
 
 
@@ -126,6 +128,27 @@ def test_batch_matches_single_runs():
         assert crc(model) == crc(single), exp.__name__
         assert np.array_equal(model.gpu.arrays.colors.get(),
                               single.gpu.arrays.colors.get())
+
+
+@pytest.mark.parametrize("tile", TUNE_TILES)
+@pytest.mark.parametrize("depth", [1, 3, 8])
+def test_batch_split_loop_every_tile(tile, depth):
+    """The split step loop (column-owning threads, ring of halo cells)
+    against the generic loop, for every tile the autotuner tries, with a
+    step count that leaves a partial launch at the end."""
+    exps = batch_experiments(count=5, seed=depth, side_min=9, side_max=150)
+    fast = Batch([LifeLikeRule(exp) for exp in exps], tile=tile, depth=depth)
+    slow = Batch([LifeLikeRule(exp) for exp in exps], tile=tile, depth=depth,
+                 split=False)
+    assert fast.program.split and not slow.program.split
+    for batch in (fast, slow):
+        batch.run(2 * depth + 1, graph=False)
+        batch.synchronize()
+        batch.scatter()
+    for a, b in zip(fast.models, slow.models):
+        assert crc(a) == crc(b)
+        assert np.array_equal(a.gpu.arrays.colors.get(),
+                              b.gpu.arrays.colors.get())
 
 
 def test_batch_graph_replays():

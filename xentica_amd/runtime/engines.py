@@ -11,6 +11,10 @@ Step engines: how a model's step is executed on the GPU.
 * ``fused_unfolded`` -- ``fused`` with buffer reads kept on LDS buffer
   planes even when emit only copies the state (the general form of the
   kernel; kept selectable to check the folded one against it).
+* ``fused_generic`` -- ``fused`` with every cell of a step going through
+  one generic loop over the valid rectangle (the form used for tiles whose
+  rows do not split evenly over the block; kept selectable to check the
+  split form against it).
 * ``auto`` -- ``fused`` when the model qualifies, else ``plain``.
 
 Engines share one interface: ``step()`` (advance one step, possibly
@@ -88,6 +92,11 @@ def _fused_unfolded(model):
     return FusedEngine(model, fold=False)
 
 
+def _fused_generic(model):
+    from xentica_amd.runtime.fused import FusedEngine
+    return FusedEngine(model, split=False)
+
+
 def _auto(model):
     """The fused engine when the model qualifies, else the plain one."""
     from xentica_amd.runtime.fused import FusedEngine, eligible
@@ -104,6 +113,7 @@ ENGINES = {
     'plain': PlainEngine,
     'fused': _fused,
     'fused_unfolded': _fused_unfolded,
+    'fused_generic': _fused_generic,
     'auto': _auto,
 }
 

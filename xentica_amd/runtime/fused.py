@@ -82,11 +82,11 @@ class FusedProgram:
     CANDIDATES = (((32, 32), 4), ((64, 16), 4), ((32, 16), 4), ((32, 32), 2),
                   ((32, 16), 2), ((16, 16), 2), ((16, 16), 1), ((8, 8), 1))
 
-    def __init__(self, model, tile=None, depth=None, fold=True):
+    def __init__(self, model, tile=None, depth=None, fold=True, split=True):
         if tile is None or depth is None:
             for cand_tile, cand_depth in self.CANDIDATES:
                 try:
-                    self.__init__(model, cand_tile, cand_depth, fold)
+                    self.__init__(model, cand_tile, cand_depth, fold, split)
                     return
                 except _DoesNotFit:
                     continue
@@ -141,6 +141,12 @@ class FusedProgram:
         # skipped when the colour code leaves the state alone
         self.main_copy = not access.absorb_writes_main
         self.color_guard = access.color_is_pure
+        # output tile rows split evenly over the block: column-owning
+        # threads with compile-time region offsets (see _STEP_LOOP_SPLIT)
+        groups = BLOCK // self.tile[0] if BLOCK % self.tile[0] == 0 else 0
+        self.split = (split and self.folded and groups > 0
+                      and self.tile[1] % groups == 0)
+        self.rows_per_thread = self.tile[1] // groups if self.split else 0
         if self.color_guard:
             absorb_body = (access.absorb_code + "if (_inner) {\n"
                            + access.color_code + "}\n" + access.store_code)
@@ -182,16 +188,29 @@ class FusedProgram:
                 "_word *_tb = 0;\n"
             emit_section = _EMIT_SECTION.format(
                 static_emit=static_emit, emit_body=emit_body)
+        main_copy = ("                _mp[_cur ^ 1][_li] = _mp[_cur][_li];"
+                     if self.main_copy else "")
+        if self.split:
+            cells = "".join(
+                _SPLIT_CELL.format(j=j, main_copy=main_copy,
+                                   absorb_body=absorb_body)
+                for j in range(self.rows_per_thread))
+            step_body = _SPLIT_STEP.format(
+                main_copy=main_copy, absorb_body=absorb_body,
+                inner_cells=cells)
+            step_loop = _STEP_LOOP_SPLIT.format(step_body=step_body)
+        else:
+            step_loop = _STEP_LOOP_GENERIC.format(
+                emit_section=emit_section, main_copy=main_copy,
+                absorb_body=absorb_body)
         return _KERNEL_TEMPLATE.format(
             defines=defines, tx=tx, ty=ty, margin=margin,
-            k=0 if self.folded else self.k,
+            k=0 if self.folded else self.k, nt=BLOCK,
+            rows=self.rows_per_thread,
             he=self.halo_per_step, ctype=self.ctype, static=border,
             bvalue="%dULL" % (self.border_value & 0xFFFFFFFFFFFFFFFF),
             params_struct=params_struct, param_locals=param_locals,
-            zero_slots=zero_slots, tb_decl=tb_decl,
-            emit_section=emit_section, absorb_body=absorb_body,
-            main_copy=("                _mp[_cur ^ 1][_li] = _mp[_cur][_li];"
-                       if self.main_copy else ""))
+            zero_slots=zero_slots, tb_decl=tb_decl, step_loop=step_loop)
 
     @property
     def function(self):
@@ -214,6 +233,8 @@ _KERNEL_TEMPLATE = r"""
 #define _RH (_TY + 2 * _M)
 #define _RA (_RW * _RH)
 #define _K {k}
+#define _NT {nt}
+#define _J {rows}
 #define _HE {he}
 #define _STATIC {static}
 #define _BVALUE {bvalue}
@@ -282,7 +303,24 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
                 _col[_q] = _cg[(long long) _gy * _in.w0 + _gx];
         }}
         __syncthreads();
-        int _cur = 0;
+{step_loop}        for (int _q = threadIdx.x; _q < _TX * _TY; _q += blockDim.x) {{
+            const int _qx = _q % _TX, _qy = _q / _TX;
+            const int _gx = _ox + _qx, _gy = _oy + _qy;
+            if (_gx < _in.w0 && _gy < _in.w1) {{
+                const long long _g = (long long) _gy * _in.w0 + _gx;
+                _dst[_g] = _mp[_cur][(_qy + _M) * _RW + _qx + _M];
+                _cg[_g] = _col[_q];
+            }}
+        }}
+        __syncthreads();
+    }}
+}}
+"""
+
+
+# The step loop of the general form: one pass per step over the valid
+# rectangle, any tile shape, with or without the emit pass.
+_STEP_LOOP_GENERIC = r"""        int _cur = 0;
         for (int _s = 0; _s < _steps; ++_s) {{
             const int _da = (_s + 1) * _HE;
 {emit_section}
@@ -307,18 +345,102 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
             __syncthreads();
             _cur ^= 1;
         }}
-        for (int _q = threadIdx.x; _q < _TX * _TY; _q += blockDim.x) {{
-            const int _qx = _q % _TX, _qy = _q / _TX;
-            const int _gx = _ox + _qx, _gy = _oy + _qy;
-            if (_gx < _in.w0 && _gy < _in.w1) {{
-                const long long _g = (long long) _gy * _in.w0 + _gx;
-                _dst[_g] = _mp[_cur][(_qy + _M) * _RW + _qx + _M];
-                _cg[_g] = _col[_q];
+"""
+
+# The step loop of a folded program whose tile rows split evenly over the
+# block: the steps come in pairs with the ping-pong plane fixed in each
+# half (_cur is a constant there), the output tile is computed by threads
+# that each own _J cells of one column (region offsets known at compile
+# time, neighbour words shared by a column's cells), and only the ring of
+# halo cells still valid at this step goes through the generic index
+# arithmetic.
+_STEP_LOOP_SPLIT = r"""
+        int _cur = 0;
+        for (int _s = 0; _s < _steps; _s += 2) {{
+            {{ const int _cur = 0, _da = (_s + 1) * _HE;
+{step_body}
             }}
+            __syncthreads();
+            if (_s + 1 >= _steps) break;
+            {{ const int _cur = 1, _da = (_s + 2) * _HE;
+{step_body}
+            }}
+            __syncthreads();
         }}
-        __syncthreads();
-    }}
-}}
+        _cur = _steps & 1;
+"""
+
+# One step of the split loop: the halo ring, then the output tile.
+_SPLIT_STEP = r"""
+            {{ const int _bw = _M - _da;
+            if (_bw > 0) {{
+                // ring = rows [_da, _M) and [_RH - _M, _RH - _da) at full
+                // width, then columns [_da, _M) and [_RW - _M, _RW - _da) of
+                // the _TY output rows; rows from the float reciprocal as in
+                // _RECT_FOR (same bound: _q < _RA, divisor <= _RW)
+                const int _ws = _RW - 2 * _da, _nb = 2 * _bw * _ws;
+                const int _nq = _nb + 2 * _bw * _TY;
+                const float _iw = 1.0f / (float) _ws, _ib = 1.0f / (float) (2 * _bw);
+                for (int _q = threadIdx.x; _q < _nq; _q += _NT) {{
+                    int _lx, _ly;
+                    if (_q < _nb) {{
+                        const int _r = (int) (((float) _q + 0.5f) * _iw);
+                        _lx = _da + (_q - _r * _ws);
+                        _ly = _r < _bw ? _da + _r : _RH - _M - _bw + _r;
+                    }} else {{
+                        const int _q2 = _q - _nb;
+                        const int _r = (int) (((float) _q2 + 0.5f) * _ib);
+                        const int _c = _q2 - _r * 2 * _bw;
+                        _ly = _M + _r;
+                        _lx = _c < _bw ? _da + _c : _RW - _M - _bw + _c;
+                    }}
+                    const int _li = _ly * _RW + _lx;
+#if _STATIC
+                    const int _gx = _ox - _M + _lx, _gy = _oy - _M + _ly;
+                    if (_gx < 0 || _gx >= _in.w0 || _gy < 0 || _gy >= _in.w1) {{
+                        _mp[_cur ^ 1][_li] = (_word) _BVALUE;
+                        continue;
+                    }}
+#endif
+{main_copy}
+                    const bool _inner = false;
+                    const int _oi = _TX * _TY + (int) threadIdx.x;
+                    {{
+                        int3 *col = &_col[_oi];
+                        const unsigned i = 0;
+{absorb_body}
+                    }}
+                }}
+            }} }}
+            {{ // output tile: column _cx, rows _cy .. _cy + _J - 1
+                const int _cx = (int) threadIdx.x % _TX;
+                const int _cy = (int) threadIdx.x / _TX * _J;
+                const int _li0 = (_M + _cy) * _RW + _M + _cx;
+                const int _oi0 = _cy * _TX + _cx;
+#if _STATIC
+                // cells past a field's far edge hold the border word; they
+                // are computed like the others and then overwritten
+                const bool _offx = _ox + _cx >= _in.w0;
+#endif
+{inner_cells}
+            }}
+"""
+
+# One cell of the output tile (row _j of the thread's column).
+_SPLIT_CELL = r"""
+                {{ const int _li = _li0 + {j} * _RW, _oi = _oi0 + {j} * _TX;
+                    const bool _inner = true;
+{main_copy}
+                    {{
+                        int3 *col = &_col[_oi];
+                        const unsigned i = 0;
+{absorb_body}
+                    }}
+#if _STATIC
+                    if (_offx || _oy + _cy + {j} >= _in.w1)
+                        _mp[_cur ^ 1][_li] = (_word) _BVALUE;
+#endif
+                }}
 """
 
 
@@ -411,9 +533,10 @@ class FusedEngine:
 
     name = "fused"
 
-    def __init__(self, model, tile=None, depth=None, fold=True):
+    def __init__(self, model, tile=None, depth=None, fold=True, split=True):
         self.model = model
-        self.program = FusedProgram(model, tile=tile, depth=depth, fold=fold)
+        self.program = FusedProgram(model, tile=tile, depth=depth, fold=fold,
+                                    split=split)
         self._inst = None
         self._pending = 0
         self._pending_values = None
@@ -500,7 +623,7 @@ class Batch:
     """
 
     def __init__(self, models, tile=(32, 32), depth=4, stream=None,
-                 fold=True):
+                 fold=True, split=True):
         models = list(models)
         if not models:
             raise XenticaException("empty batch")
@@ -508,7 +631,7 @@ class Batch:
         if any(type(m) is not cls for m in models):
             raise XenticaException("a batch holds instances of one class")
         self.program = FusedProgram(models[0], tile=tile, depth=depth,
-                                    fold=fold)
+                                    fold=fold, split=split)
         self.models = models
         self.stream = stream or hip.Stream()
         self.inst = _Instances(self.program, models)
