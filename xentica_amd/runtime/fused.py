@@ -147,6 +147,8 @@ class FusedProgram:
         self.split = (split and self.folded and groups > 0
                       and self.tile[1] % groups == 0)
         self.rows_per_thread = self.tile[1] // groups if self.split else 0
+        # cells per ring run unrolled in the kernel (longer runs loop)
+        self.ring_run = max(1, min(self.halo_per_step * self.depth - 1, 8))
         if self.color_guard:
             absorb_body = (access.absorb_code + "if (_inner) {\n"
                            + access.color_code + "}\n" + access.store_code)
@@ -195,9 +197,11 @@ class FusedProgram:
                 _SPLIT_CELL.format(j=j, main_copy=main_copy,
                                    absorb_body=absorb_body)
                 for j in range(self.rows_per_thread))
+            run = self.ring_run
             step_body = _SPLIT_STEP.format(
-                main_copy=main_copy, absorb_body=absorb_body,
-                inner_cells=cells)
+                inner_cells=cells,
+                vertical_run=_ring_run("_RW", run, main_copy, absorb_body),
+                horizontal_run=_ring_run("1", run, main_copy, absorb_body))
             step_loop = _STEP_LOOP_SPLIT.format(step_body=step_body)
         else:
             step_loop = _STEP_LOOP_GENERIC.format(
@@ -206,7 +210,7 @@ class FusedProgram:
         return _KERNEL_TEMPLATE.format(
             defines=defines, tx=tx, ty=ty, margin=margin,
             k=0 if self.folded else self.k, nt=BLOCK,
-            rows=self.rows_per_thread,
+            rows=self.rows_per_thread, run=self.ring_run,
             he=self.halo_per_step, ctype=self.ctype, static=border,
             bvalue="%dULL" % (self.border_value & 0xFFFFFFFFFFFFFFFF),
             params_struct=params_struct, param_locals=param_locals,
@@ -232,9 +236,11 @@ _KERNEL_TEMPLATE = r"""
 #define _RW (_TX + 2 * _M)
 #define _RH (_TY + 2 * _M)
 #define _RA (_RW * _RH)
+#define _RP (_NT / _RW)
 #define _K {k}
 #define _NT {nt}
 #define _J {rows}
+#define _RUN {run}
 #define _HE {he}
 #define _STATIC {static}
 #define _BVALUE {bvalue}
@@ -263,6 +269,11 @@ int tiles_x;
 int tile0;
 {params_struct}}};
 
+// element index of in-field cell (x, y) of the current instance: both
+// coordinates are in range, so the unsigned 32x32->64-bit product is exact
+#define _index(y, x) ((unsigned long long) (unsigned) (y) * (unsigned) _in.w0 \
+                      + (unsigned) (x))
+
 // torus wrap; only coordinates outside [0, n) (tiles on a field's edge)
 // pay for the division
 __device__ __forceinline__ int _wrap(int v, int n) {{
@@ -287,31 +298,112 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
         _word *_dst = _dst_all + _in.cells;
         int3 *_cg = _col_all + _in.cols;
         {param_locals}
-        for (int _q = threadIdx.x; _q < _RA; _q += blockDim.x) {{
+#if _RW <= _NT
+        {{ // thread (column _c, phase _p) loads region rows _p, _p + _RP, ...,
+           // eight loads in flight before the LDS writes; the column's wrap
+           // is resolved once per tile
+            const int _c = (int) threadIdx.x % _RW, _p = (int) threadIdx.x / _RW;
+            if (_p < _RP) {{
+#if _STATIC
+                const int _gx = _ox - _M + _c;
+                const bool _offx = _gx < 0 || _gx >= _in.w0;
+#else
+                const int _gx = _wrap(_ox - _M + _c, _in.w0);
+#endif
+                for (int _r0 = _p; _r0 < _RH; _r0 += 8 * _RP) {{
+                    _word _v[8];
+#pragma unroll
+                    for (int _k = 0; _k < 8; ++_k) {{
+                        const int _r = _r0 + _k * _RP, _gy = _oy - _M + _r;
+                        if (_r < _RH) {{
+#if _STATIC
+                            const bool _off = _offx || _gy < 0 || _gy >= _in.w1;
+                            _v[_k] = _off ? (_word) _BVALUE : _src[_index(_gy, _gx)];
+#else
+                            _v[_k] = _src[_index(_wrap(_gy, _in.w1), _gx)];
+#endif
+                        }}
+                    }}
+#pragma unroll
+                    for (int _k = 0; _k < 8; ++_k) {{
+                        const int _r = _r0 + _k * _RP;
+                        if (_r < _RH) {{
+                            const int _q = _r * _RW + _c;
+                            _mp[0][_q] = _v[_k];
+                            {zero_slots}
+                        }}
+                    }}
+                }}
+            }}
+        }}
+#else
+        for (int _q = threadIdx.x; _q < _RA; _q += _NT) {{
             const int _gx = _ox - _M + _q % _RW, _gy = _oy - _M + _q / _RW;
 #if _STATIC
             const bool _off = _gx < 0 || _gx >= _in.w0 || _gy < 0 || _gy >= _in.w1;
-            _mp[0][_q] = _off ? (_word) _BVALUE : _src[(long long) _gy * _in.w0 + _gx];
+            _mp[0][_q] = _off ? (_word) _BVALUE : _src[_index(_gy, _gx)];
 #else
-            _mp[0][_q] = _src[(long long) _wrap(_gy, _in.w1) * _in.w0 + _wrap(_gx, _in.w0)];
+            _mp[0][_q] = _src[_index(_wrap(_gy, _in.w1), _wrap(_gx, _in.w0))];
 #endif
             {zero_slots}
         }}
-        for (int _q = threadIdx.x; _q < _TX * _TY; _q += blockDim.x) {{
+#endif
+#if _NT % _TX == 0
+        {{ // colours of the output tile: thread (column _cx, phase _ry) loads
+           // rows _ry, _ry + _NT / _TX, ... inside the field, eight at a time
+            const int _cx = (int) threadIdx.x % _TX, _gx = _ox + _cx;
+            const int _ry = (int) threadIdx.x / _TX;
+            if (_gx < _in.w0)
+                for (int _r0 = _ry; _r0 < _TY; _r0 += 8 * (_NT / _TX)) {{
+                    // three int arrays (an int3 array is not kept in registers)
+                    int _c0[8], _c1[8], _c2[8];
+#pragma unroll
+                    for (int _k = 0; _k < 8; ++_k) {{
+                        const int _r = _r0 + _k * (_NT / _TX);
+                        if (_r < _TY && _oy + _r < _in.w1) {{
+                            const int3 _cv = _cg[_index(_oy + _r, _gx)];
+                            _c0[_k] = _cv.x;
+                            _c1[_k] = _cv.y;
+                            _c2[_k] = _cv.z;
+                        }}
+                    }}
+#pragma unroll
+                    for (int _k = 0; _k < 8; ++_k) {{
+                        const int _r = _r0 + _k * (_NT / _TX);
+                        if (_r < _TY && _oy + _r < _in.w1)
+                            _col[_r * _TX + _cx] = make_int3(_c0[_k], _c1[_k], _c2[_k]);
+                    }}
+                }}
+        }}
+#else
+        for (int _q = threadIdx.x; _q < _TX * _TY; _q += _NT) {{
             const int _gx = _ox + _q % _TX, _gy = _oy + _q / _TX;
             if (_gx < _in.w0 && _gy < _in.w1)
-                _col[_q] = _cg[(long long) _gy * _in.w0 + _gx];
+                _col[_q] = _cg[_index(_gy, _gx)];
         }}
+#endif
         __syncthreads();
-{step_loop}        for (int _q = threadIdx.x; _q < _TX * _TY; _q += blockDim.x) {{
+{step_loop}#if _NT % _TX == 0
+        {{ const int _cx = (int) threadIdx.x % _TX, _gx = _ox + _cx;
+            if (_gx < _in.w0)
+                for (int _r = (int) threadIdx.x / _TX; _r < _TY && _oy + _r < _in.w1;
+                     _r += _NT / _TX) {{
+                    const unsigned long long _g = _index(_oy + _r, _gx);
+                    _dst[_g] = _mp[_cur][(_r + _M) * _RW + _cx + _M];
+                    _cg[_g] = _col[_r * _TX + _cx];
+                }}
+        }}
+#else
+        for (int _q = threadIdx.x; _q < _TX * _TY; _q += _NT) {{
             const int _qx = _q % _TX, _qy = _q / _TX;
             const int _gx = _ox + _qx, _gy = _oy + _qy;
             if (_gx < _in.w0 && _gy < _in.w1) {{
-                const long long _g = (long long) _gy * _in.w0 + _gx;
+                const unsigned long long _g = _index(_gy, _gx);
                 _dst[_g] = _mp[_cur][(_qy + _M) * _RW + _qx + _M];
                 _cg[_g] = _col[_q];
             }}
         }}
+#endif
         __syncthreads();
     }}
 }}
@@ -374,41 +466,32 @@ _STEP_LOOP_SPLIT = r"""
 _SPLIT_STEP = r"""
             {{ const int _bw = _M - _da;
             if (_bw > 0) {{
-                // ring = rows [_da, _M) and [_RH - _M, _RH - _da) at full
-                // width, then columns [_da, _M) and [_RW - _M, _RW - _da) of
-                // the _TY output rows; rows from the float reciprocal as in
-                // _RECT_FOR (same bound: _q < _RA, divisor <= _RW)
-                const int _ws = _RW - 2 * _da, _nb = 2 * _bw * _ws;
-                const int _nq = _nb + 2 * _bw * _TY;
-                const float _iw = 1.0f / (float) _ws, _ib = 1.0f / (float) (2 * _bw);
-                for (int _q = threadIdx.x; _q < _nq; _q += _NT) {{
-                    int _lx, _ly;
-                    if (_q < _nb) {{
-                        const int _r = (int) (((float) _q + 0.5f) * _iw);
-                        _lx = _da + (_q - _r * _ws);
-                        _ly = _r < _bw ? _da + _r : _RH - _M - _bw + _r;
+                // the ring of halo cells still valid at this step, as runs
+                // of _bw cells: one per column of the top band (rows
+                // [_da, _M)) and of the bottom band (rows [_RH - _M,
+                // _RH - _da)), then one per output row in the left band
+                // (columns [_da, _M)) and in the right band (columns
+                // [_RW - _M, _RW - _da)); a run's cells share their
+                // neighbour words
+                const int _ws = _RW - 2 * _da, _nv = 2 * _ws, _nu = _nv + 2 * _TY;
+                for (int _u = threadIdx.x; _u < _nu; _u += _NT) {{
+                    if (_u < _nv) {{
+                        const bool _low = _u >= _ws;
+                        const int _lx = _da + (_low ? _u - _ws : _u);
+                        const int _ly0 = _low ? _RH - _M : _da;
+                        for (int _r0 = 0; _r0 < _bw; _r0 += _RUN) {{
+                            const int _li0 = (_ly0 + _r0) * _RW + _lx, _rn = _bw - _r0;
+{vertical_run}
+                        }}
                     }} else {{
-                        const int _q2 = _q - _nb;
-                        const int _r = (int) (((float) _q2 + 0.5f) * _ib);
-                        const int _c = _q2 - _r * 2 * _bw;
-                        _ly = _M + _r;
-                        _lx = _c < _bw ? _da + _c : _RW - _M - _bw + _c;
-                    }}
-                    const int _li = _ly * _RW + _lx;
-#if _STATIC
-                    const int _gx = _ox - _M + _lx, _gy = _oy - _M + _ly;
-                    if (_gx < 0 || _gx >= _in.w0 || _gy < 0 || _gy >= _in.w1) {{
-                        _mp[_cur ^ 1][_li] = (_word) _BVALUE;
-                        continue;
-                    }}
-#endif
-{main_copy}
-                    const bool _inner = false;
-                    const int _oi = _TX * _TY + (int) threadIdx.x;
-                    {{
-                        int3 *col = &_col[_oi];
-                        const unsigned i = 0;
-{absorb_body}
+                        const int _v = _u - _nv;
+                        const bool _right = _v >= _TY;
+                        const int _ly = _M + (_right ? _v - _TY : _v);
+                        const int _lx0 = _right ? _RW - _M : _da;
+                        for (int _r0 = 0; _r0 < _bw; _r0 += _RUN) {{
+                            const int _li0 = _ly * _RW + _lx0 + _r0, _rn = _bw - _r0;
+{horizontal_run}
+                        }}
                     }}
                 }}
             }} }}
@@ -442,6 +525,40 @@ _SPLIT_CELL = r"""
 #endif
                 }}
 """
+
+
+# One ring cell (``_li`` = ``_li0 + {offset}``); colour goes to the
+# thread's spare slot.
+_RING_CELL = r"""
+                            {{ const int _li = _li0 + {offset};
+                                const bool _inner = false;
+                                const int _oi = _TX * _TY + (int) threadIdx.x;
+{main_copy}
+                                {{
+                                    int3 *col = &_col[_oi];
+                                    const unsigned i = 0;
+{absorb_body}
+                                }}
+#if _STATIC
+                                {{ const int _gx = _ox - _M + _li % _RW, _gy = _oy - _M + _li / _RW;
+                                if (_gx < 0 || _gx >= _in.w0 || _gy < 0 || _gy >= _in.w1)
+                                    _mp[_cur ^ 1][_li] = (_word) _BVALUE; }}
+#endif
+                            }}
+"""
+
+
+def _ring_run(stride, length, main_copy, absorb_body):
+    """Cells ``0 .. min(_rn, length) - 1`` of a ring run; cell ``r`` sits
+    inside ``if (_rn > r)`` after cell ``r - 1``, so that the loads of the
+    earlier cells dominate it and can be reused."""
+    def cell(r):
+        return _RING_CELL.format(offset="%d * %s" % (r, stride),
+                                 main_copy=main_copy, absorb_body=absorb_body)
+    text = ""
+    for r in reversed(range(1, length)):
+        text = "if (_rn > %d) {\n%s%s}\n" % (r, cell(r), text)
+    return cell(0) + text
 
 
 _EMIT_SECTION = r"""
