@@ -67,13 +67,21 @@ class MovingAverage(ColorEffect):
         bsca.define_constant(Constant("FADE_IN", bsca.fade_in))
         bsca.define_constant(Constant("FADE_OUT", bsca.fade_out))
         bsca.define_constant(Constant("SMOOTH_FACTOR", bsca.smooth_factor))
+        clamp = ("new_{c} = max(min(new_{c}, old_col.{x} + FADE_IN), "
+                 "old_col.{x} - FADE_OUT);\n")
+        if (isinstance(bsca.fade_in, int) and isinstance(bsca.fade_out, int)
+                and bsca.fade_in + bsca.fade_out >= 0):
+            # lo = old - FADE_OUT <= hi = old + FADE_IN, so the clamp is
+            # the median of (new, lo, hi): one v_med3_i32 instead of a
+            # min and a max
+            clamp = ("new_{c} = max(min(new_{c}, old_col.{x} - FADE_OUT), "
+                     "min(max(new_{c}, old_col.{x} - FADE_OUT), "
+                     "old_col.{x} + FADE_IN));\n")
         self.effect = (
             "new_r *= SMOOTH_FACTOR;\n"
             "new_g *= SMOOTH_FACTOR;\n"
             "new_b *= SMOOTH_FACTOR;\n"
             "int3 old_col = col[i];\n"
-            "new_r = max(min(new_r, old_col.x + FADE_IN), old_col.x - FADE_OUT);\n"
-            "new_g = max(min(new_g, old_col.y + FADE_IN), old_col.y - FADE_OUT);\n"
-            "new_b = max(min(new_b, old_col.z + FADE_IN), old_col.z - FADE_OUT);\n"
-        )
+            + "".join(clamp.format(c=c, x=x)
+                      for c, x in (("r", "x"), ("g", "y"), ("b", "z"))))
         return super().__call__()
