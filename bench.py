@@ -77,7 +77,7 @@ def autotune(field, stream, trial_steps=96):
     tiles_x = -(-(field.width // 32) // 62)
     best = None
     for depth in (5, 6, 8):
-        for waves in (2, 4, 6, 8):
+        for waves in (2, 3, 4, 6, 8):
             strips = max(1, min(field.height, -(-waves * simds // tiles_x)))
             field.depth, field.strips = depth, strips
             timed(field, stream, 2 * depth)

@@ -38,3 +38,13 @@ def test_runtime_rules(born, sustain):
 
 def test_explicit_strips():
     _check(2016, 50, 4, 8, 12, 9, strips=7)
+
+
+@pytest.mark.parametrize("height,strips", [(30, 10), (63, 9), (64, 8), (72, 8),
+                                           (75, 5), (77, 7)])
+def test_strip_heights_around_depth(height, strips):
+    """Strips of 3, 7, 8, 9, 15 and 11 rows at depth 8: prologue longer
+    than the strip, no steady-state iteration, steady state plus a drain
+    of every length modulo four."""
+    _check(64, height, 8, 8, 12, 8, strips=strips)
+    _check(64, height, 8, 6, 20, 8, strips=strips, seed=1)
