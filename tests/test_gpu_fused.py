@@ -151,6 +151,34 @@ def test_batch_split_loop_every_tile(tile, depth):
                               b.gpu.arrays.colors.get())
 
 
+@pytest.mark.parametrize("tile", TUNE_TILES)
+def test_static_border_split_loop_every_tile(tile):
+    """Static border (live cells outside the field) through the split step
+    loop at every autotuner tile: off-board cells of edge tiles, inner and
+    ring, against the plain engine run alone."""
+    from xentica_amd import seeds
+    sizes = [(70, 45), (33, 130), (9, 11)]
+    exps = [type("Static%d" % idx, (GOLExperiment, ), {
+        "size": size, "word": 5 + idx,
+        "seed": seeds.patterns.PrimordialSoup(
+            vals={"state": seeds.random.RandInt(0, 1)})})
+        for idx, size in enumerate(sizes)]
+    for depth in (3, 8):
+        batch = Batch([GameOfLifeStatic(exp) for exp in exps], tile=tile,
+                      depth=depth)
+        assert batch.program.split and batch.program.static
+        batch.run(2 * depth + 1, graph=False)
+        batch.synchronize()
+        batch.scatter()
+        for exp, model in zip(exps, batch.models):
+            plain = run(GameOfLifeStatic(
+                type("Plain" + exp.__name__, (exp, ), {"engine": "plain"})),
+                2 * depth + 1)
+            assert crc(model) == crc(plain), (exp.__name__, depth)
+            assert np.array_equal(model.gpu.arrays.colors.get(),
+                                  plain.gpu.arrays.colors.get())
+
+
 def test_batch_graph_replays():
     """The captured graph replayed several times equals eager launches."""
     exps = batch_experiments(count=6, seed=3, side_min=40, side_max=200)

@@ -841,7 +841,7 @@ class Batch:
 #: Tile shapes and temporal depths the autotuner tries (those that fit the
 #: LDS budget for the model's word size and fold).
 TUNE_TILES = ((32, 32), (64, 16), (32, 16), (64, 32), (128, 8), (16, 16))
-TUNE_DEPTHS = (1, 2, 4, 6, 8)
+TUNE_DEPTHS = (1, 2, 3, 4, 6, 8)
 
 
 def autotune(models, trial_steps=64, tiles=TUNE_TILES, depths=TUNE_DEPTHS,
