@@ -196,7 +196,7 @@ def batched(steps=1000, plain_steps=10):
     }
 
 
-def life3d(steps=200):
+def life3d(steps=200, cpu=True):
     """3-D B5/S45, 1024^3 torus, 10 % density: packed vs unpacked engines."""
     from examples.workloads import life3d_cells
     from xentica_amd.runtime import hip
@@ -228,10 +228,53 @@ def life3d(steps=200):
                      "algorithmic_GBps": value * bpc / 1e9,
                      "roofline_frac": value * bpc / 1e9 / HBM_PEAK_GBS}
         del field
+    if cpu:
+        out["cpu_baseline"] = life3d_cpu_baseline(born, sustain)
     return out
 
 
-def energy_life(steps=5000, plain_steps=20):
+def life3d_cpu_baseline(born, sustain, side=256, steps=10):
+    """
+    The oracle on a 256^3 sample of the same workload (the reference's
+    plain layout of 1024^3 would need 27 GiB of buffer planes): same rule,
+    26-offset list and 10 % seeding, ``steps`` steps after one warm-up.
+    """
+    from examples.workloads import life3d_cells
+    from oracle import oracle
+    cores = len(os.sched_getaffinity(0))
+    size = (side, side, side)
+    field = oracle.Field(size, 26, np.uint8, life3d_cells(size))
+    args = (oracle.moore_deltas(3), oracle.moore_rev(3), born, sustain)
+    oracle.lifelike(field, *args, 1, colors=False, threads=cores)
+    t0 = time.perf_counter()
+    oracle.lifelike(field, *args, steps, colors=False, threads=cores)
+    elapsed = time.perf_counter() - t0
+    return {"value": side ** 3 * steps / elapsed, "unit": "cell-updates/s",
+            "cores": cores, "kind": "port",
+            "sample": "%d^3 torus (the 1024^3 field in the reference's plain "
+                      "layout needs 27 GiB), B5/S45 with the 26-offset list, "
+                      "10 %% density, %d steps after 1 warm-up step, 26 buffer "
+                      "planes, no colours, C + OpenMP" % (side, steps)}
+
+
+def energy_cpu_baseline(steps=20):
+    """The oracle on the same 4096^2 EnergyLife field, ``steps`` steps
+    (colours included, as the fused engine computes them)."""
+    from oracle import oracle
+    cores = len(os.sched_getaffinity(0))
+    field = oracle.energy_life_field((4096, 4096))
+    oracle.energy_life(field, 1, threads=cores)
+    t0 = time.perf_counter()
+    oracle.energy_life(field, steps, threads=cores)
+    elapsed = time.perf_counter() - t0
+    return {"value": 4096 * 4096 * steps / elapsed, "unit": "cell-updates/s",
+            "cores": cores, "kind": "port",
+            "sample": "EnergyLife 4096x4096 word 42, %d steps after 1 warm-up "
+                      "step, reference plain emit/absorb structure (8 buffer "
+                      "planes of 128-bit words, colours), C + OpenMP" % steps}
+
+
+def energy_life(steps=5000, plain_steps=20, cpu=True):
     """
     EnergyLife on a 4096x4096 torus: 128-bit words (energy 16 bits, genome
     18, mode 1, xorshift32 32), word 42, ``steps`` steps by the fused
@@ -283,6 +326,7 @@ def energy_life(steps=5000, plain_steps=20):
         "alive_fraction_end": float(mode.mean()),
         "plain_loop_value": plain_value, "plain_loop_steps": plain_steps,
         "speedup_vs_plain": value / plain_value,
+        **({"cpu_baseline": energy_cpu_baseline()} if cpu else {}),
     }
 
 
@@ -445,7 +489,9 @@ def main():
         return 2
     if args.only:
         log("%s only ..." % args.only)
-        print(json.dumps({args.only: _LEGS[args.only]()}), flush=True)
+        kwargs = ({"cpu": not args.no_cpu}
+                  if args.only in ("life3d", "energy") else {})
+        print(json.dumps({args.only: _LEGS[args.only](**kwargs)}), flush=True)
         return 0
 
     from xentica_amd.runtime import hip
@@ -508,10 +554,10 @@ def main():
         result["speedup_vs_plain"] = value / result["plain_engine"]["value"]
     if not args.no_3d:
         log("3-D workload ...")
-        result["life3d"] = life3d()
+        result["life3d"] = life3d(cpu=not args.no_cpu)
     if not args.no_energy:
         log("EnergyLife workload ...")
-        result["energy_life"] = energy_life()
+        result["energy_life"] = energy_life(cpu=not args.no_cpu)
     if not args.no_batch:
         log("batched workload ...")
         result["batched"] = batched()
