@@ -356,7 +356,7 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
     from examples.sand_heights import SandHeights, sand_experiment
     from xentica_amd.runtime.fused import autotune
     from xentica_amd.surrogate.network import (
-        NCASurrogate, NCAFunction, encode, forward_raw)
+        NCASurrogate, NCAFunction, encode, forward_raw, predict)
     from xentica_amd.surrogate.train import (
         pairs_from_model, train, evaluate, value_mask)
     size = (4096, 4096)
@@ -391,11 +391,16 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
         batch.run(16)
         encode(batch.state[batch.cur].ptr, 1, h, w, x, mean, mask, stream)
 
-    def fast():
+    def two_launches():
         encode(batch.state[batch.cur].ptr, 1, h, w, x, None, mask, stream)
         forward_raw(x, params, y, pred, stream)
 
+    def fast():
+        predict(batch.state[batch.cur].ptr, 1, h, w, params, None, pred, mask,
+                stream)
+
     exact_ms = _event_ms(stream, exact, repeats)
+    two_ms = _event_ms(stream, two_launches, repeats)
     fast_ms = _event_ms(stream, fast, repeats)
 
     # the reference-structured plain engine for the same 16 steps
@@ -440,8 +445,11 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
                   "loss_last": losses[-1]},
         "heldout": score,
         "time_to_coarse_field_ms": {"surrogate": fast_ms,
+                                    "surrogate_two_launches": two_ms,
                                     "exact_fused": exact_ms,
                                     "exact_plain": plain_ms},
+        "surrogate_path": "xen_nca_predict: fine field to block means in one "
+                          "launch (encode into LDS, then the forward)",
         "speedup_vs_exact": exact_ms / fast_ms,
         "speedup_vs_plain": plain_ms / fast_ms,
         "exact_engine": "fused, tile %s depth %d, graph" % (

@@ -46,6 +46,14 @@ int xen_nca_forward(const float *x, long long batch, int h, int w, const float *
                     const float *bdw, const uint16_t *w1, const float *b1, const uint16_t *w2,
                     const float *b2, float *y, float *pred, xen_stream stream);
 
+/* xen_nca_encode followed by xen_nca_forward in one launch, without x:
+ * y and pred (either may be NULL) equal theirs bit for bit.  h must be a
+ * multiple of 8 and w of 16. */
+int xen_nca_predict(const uint8_t *fine, long long batch, int h, int w, uint32_t value_mask,
+                    const float *dw, const float *bdw, const uint16_t *w1, const float *b1,
+                    const uint16_t *w2, const float *b2, float *y, float *pred,
+                    xen_stream stream);
+
 /* Rows of the fp32 workspace xen_nca_backward needs (XEN_NCA_PARAMS
  * floats each), or -1 for an invalid grid. */
 long long xen_nca_partial_rows(long long batch, int h, int w);

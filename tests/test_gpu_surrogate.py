@@ -50,6 +50,44 @@ def test_encode_matches_oracle():
     assert np.array_equal(mean.cpu().numpy(), want_mean.astype(np.float32))
 
 
+@pytest.mark.parametrize("batch,h,w", [(2, 16, 32), (3, 8, 16), (1, 24, 48)])
+def test_predict_equals_encode_then_forward(batch, h, w):
+    """The one-launch path (encode into LDS with a torus halo, then the
+    forward) gives y and pred bit for bit as the two launches do; the
+    (8, 16) grid is one tile whose halo wraps onto itself."""
+    rng = np.random.default_rng(batch * 100 + h)
+    fine = rng.integers(0, 256, (batch, 16 * h, 16 * w), dtype=np.uint8)
+    dev = torch.from_numpy(fine).cuda()
+    net = NCASurrogate(seed=h)
+    net.set_normalization(4.0, 2.5)
+    params = net.inference_params()
+    x = torch.empty((batch, h, w, 16), device="cuda")
+    y_ref = torch.empty_like(x)
+    pred_ref = torch.empty((batch, h, w), device="cuda")
+    network.encode(dev.data_ptr(), batch, h, w, x, None, value_mask=0x0F)
+    network.forward_raw(x, params, y_ref, pred_ref)
+    y = torch.full_like(x, float("nan"))
+    pred = torch.full_like(pred_ref, float("nan"))
+    network.predict(dev.data_ptr(), batch, h, w, params, y, pred,
+                    value_mask=0x0F)
+    pred_only = torch.full_like(pred_ref, float("nan"))
+    network.predict(dev.data_ptr(), batch, h, w, params, None, pred_only,
+                    value_mask=0x0F)
+    torch.cuda.synchronize()
+    assert torch.equal(y, y_ref)
+    assert torch.equal(pred, pred_ref)
+    assert torch.equal(pred_only, pred_ref)
+
+
+def test_predict_rejects_grids_off_its_tile():
+    from xentica_amd.core.exceptions import XenticaException
+    dev = torch.zeros((1, 16 * 12, 16 * 16), dtype=torch.uint8, device="cuda")
+    params = NCASurrogate(seed=1).inference_params()
+    pred = torch.empty((1, 12, 16), device="cuda")
+    with pytest.raises(XenticaException, match="multiple of 8"):
+        network.predict(dev.data_ptr(), 1, 12, 16, params, None, pred)
+
+
 def test_forward_matches_oracle():
     rng = np.random.default_rng(12)
     x = rng.uniform(0, 8, (2, 8, 12, 16)).astype(np.float32)

@@ -171,6 +171,37 @@ __device__ __forceinline__ f32x4 load_bias(const float *__restrict__ b, int off)
     return r;
 }
 
+// The 1x1 MLP of one 16-cell chunk from its bf16 perception (lane: cell m,
+// channels 4g..4g+3), and the stores of y and pred (either may be null).
+__device__ __forceinline__ f32x4 mlp(bf16x4 pb, int g, int m, const short *__restrict__ w1,
+                                     const float *__restrict__ b1, const short *__restrict__ w2,
+                                     const float *__restrict__ b2) {
+    f32x4 out = load_bias(b2, 4 * g);
+#pragma unroll
+    for (int blk = 0; blk < 4; ++blk) {
+        const bf16x4 a1 = *reinterpret_cast<const bf16x4 *>(w1 + (16 * blk + m) * kC + 4 * g);
+        const f32x4 z = mfma(a1, pb, load_bias(b1, 16 * blk + 4 * g));
+        const bf16x4 hb = pack4(fmaxf(z[0], 0.f), fmaxf(z[1], 0.f), fmaxf(z[2], 0.f),
+                                fmaxf(z[3], 0.f));
+        const bf16x4 a2 = *reinterpret_cast<const bf16x4 *>(w2 + m * kHid + 16 * blk + 4 * g);
+        out = mfma(a2, hb, out);
+    }
+    return out;
+}
+
+__device__ __forceinline__ void store_out(f32x4 out, long long index, int g,
+                                          float *__restrict__ y, float *__restrict__ pred) {
+    if (y)
+        *reinterpret_cast<float4 *>(y + index * kC + 4 * g) =
+            make_float4(out[0], out[1], out[2], out[3]);
+    if (pred) {
+        float s = (out[0] + out[1]) + (out[2] + out[3]);
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        if (g == 0) pred[index] = s * (1.0f / 16);
+    }
+}
+
 // One wavefront per chunk of 16 coarse cells, grid-stride over chunks.
 __global__ __launch_bounds__(kBlock) void forward_kernel(
         const float *__restrict__ x, long long chunks, int h, int w,
@@ -188,24 +219,78 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(
         float4 nb[kTaps];
         perceive(x, c, g, h, w, dw, bdw, p, nb);
         const bf16x4 pb = pack4(p[0], p[1], p[2], p[3]);
-        f32x4 out = load_bias(b2, 4 * g);
+        store_out(mlp(pb, g, m, w1, b1, w2, b2), c.index, g, y, pred);
+    }
+}
+
+// Fine field to prediction in one launch (the time-to-coarse-field path).
+// A workgroup owns kPredH x 16 coarse cells: it encodes them and their
+// one-cell torus halo into LDS (work item = coarse cell x sub-block row,
+// four 16-byte fine rows each), then every wave runs the forward of whole
+// tile rows (16 cells = one MFMA chunk) from LDS.  The sums, the tap order
+// of the perception and the MLP are those of encode_kernel and
+// forward_kernel, so y and pred equal theirs bit for bit.
+constexpr int kPredH = 8;
+constexpr int kPredW = 16;
+
+__global__ __launch_bounds__(kBlock) void predict_kernel(
+        const uint8_t *__restrict__ fine, long long batch, int h, int w, uint32_t mask,
+        const float *__restrict__ dw, const float *__restrict__ bdw,
+        const short *__restrict__ w1, const float *__restrict__ b1,
+        const short *__restrict__ w2, const float *__restrict__ b2,
+        float *__restrict__ y, float *__restrict__ pred) {
+    constexpr int RW = kPredW + 2, RH = kPredH + 2, RC = RW * RH;
+    __shared__ float4 xs[RC * 4];  // [region cell][channel group]
+    const int tiles_x = w / kPredW;
+    const long long tiles = static_cast<long long>(tiles_x) * (h / kPredH);
+    const long long fine_w = 16LL * w;
+    const uint32_t mask4 = mask * 0x01010101u;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, m = lane & 15;
+    for (long long t = blockIdx.x; t < batch * tiles; t += gridDim.x) {
+        const long long b = t / tiles;
+        const int tt = static_cast<int>(t % tiles);
+        const int ty0 = (tt / tiles_x) * kPredH, tx0 = (tt % tiles_x) * kPredW;
+        for (int it = threadIdx.x; it < RC * 4; it += kBlock) {
+            const int rc = it >> 2, sy = it & 3;
+            const int ry = rc / RW, rx = rc - ry * RW;
+            int cy = ty0 - 1 + ry, cx = tx0 - 1 + rx;
+            cy = cy < 0 ? cy + h : (cy >= h ? cy - h : cy);
+            cx = cx < 0 ? cx + w : (cx >= w ? cx - w : cx);
+            const uint8_t *base = fine + ((b * h + cy) * 16 + 4 * sy) * fine_w + 16LL * cx;
+            uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
-        for (int blk = 0; blk < 4; ++blk) {
-            const bf16x4 a1 = *reinterpret_cast<const bf16x4 *>(w1 + (16 * blk + m) * kC + 4 * g);
-            const f32x4 z = mfma(a1, pb, load_bias(b1, 16 * blk + 4 * g));
-            const bf16x4 hb = pack4(fmaxf(z[0], 0.f), fmaxf(z[1], 0.f), fmaxf(z[2], 0.f),
-                                    fmaxf(z[3], 0.f));
-            const bf16x4 a2 = *reinterpret_cast<const bf16x4 *>(w2 + m * kHid + 16 * blk + 4 * g);
-            out = mfma(a2, hb, out);
+            for (int r = 0; r < 4; ++r) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(base + r * fine_w);
+                s0 += byte_sum(v.x & mask4);
+                s1 += byte_sum(v.y & mask4);
+                s2 += byte_sum(v.z & mask4);
+                s3 += byte_sum(v.w & mask4);
+            }
+            xs[rc * 4 + sy] = make_float4(s0 * (1.0f / 16), s1 * (1.0f / 16),
+                                          s2 * (1.0f / 16), s3 * (1.0f / 16));
         }
-        *reinterpret_cast<float4 *>(y + c.index * kC + 4 * g) =
-            make_float4(out[0], out[1], out[2], out[3]);
-        if (pred) {
-            float s = (out[0] + out[1]) + (out[2] + out[3]);
-            s += __shfl_xor(s, 16);
-            s += __shfl_xor(s, 32);
-            if (g == 0) pred[c.index] = s * (1.0f / 16);
+        __syncthreads();
+        for (int r = wave; r < kPredH; r += kWaves) {
+            float p[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) p[i] = bdw[4 * g + i];
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int tap = (dy + 1) * 3 + (dx + 1);
+                    const float4 v = xs[((r + 1 + dy) * RW + (m + 1 + dx)) * 4 + g];
+                    p[0] = fmaf(dw[(4 * g + 0) * kTaps + tap], v.x, p[0]);
+                    p[1] = fmaf(dw[(4 * g + 1) * kTaps + tap], v.y, p[1]);
+                    p[2] = fmaf(dw[(4 * g + 2) * kTaps + tap], v.z, p[2]);
+                    p[3] = fmaf(dw[(4 * g + 3) * kTaps + tap], v.w, p[3]);
+                }
+            const bf16x4 pb = pack4(p[0], p[1], p[2], p[3]);
+            const long long index = (b * h + ty0 + r) * static_cast<long long>(w) + tx0 + m;
+            store_out(mlp(pb, g, m, w1, b1, w2, b2), index, g, y, pred);
         }
+        __syncthreads();
     }
 }
 
@@ -441,6 +526,25 @@ int xen_nca_forward(const float *x, long long batch, int h, int w, const float *
                        reinterpret_cast<const short *>(w1), b1,
                        reinterpret_cast<const short *>(w2), b2, y, pred);
     return xen::check(hipGetLastError(), "forward_kernel launch");
+}
+
+int xen_nca_predict(const uint8_t *fine, long long batch, int h, int w, uint32_t value_mask,
+                    const float *dw, const float *bdw, const uint16_t *w1, const float *b1,
+                    const uint16_t *w2, const float *b2, float *y, float *pred,
+                    xen_stream stream) {
+    if (batch < 1 || h < 1 || w < 1) return xen::fail("xen_nca_predict: empty grid", 22);
+    if (h % kPredH || w % kPredW)
+        return xen::fail("xen_nca_predict: h must be a multiple of 8 and w of 16", 22);
+    if (value_mask > 255) return xen::fail("xen_nca_predict: value_mask must fit a byte", 22);
+    if (batch * h * static_cast<long long>(w) > (1LL << 40))
+        return xen::fail("xen_nca_predict: too many coarse cells", 22);
+    const long long tiles = batch * (h / kPredH) * static_cast<long long>(w / kPredW);
+    const unsigned groups = static_cast<unsigned>(tiles < 256LL * 8 ? tiles : 256LL * 8);
+    hipLaunchKernelGGL(predict_kernel, dim3(groups), dim3(kBlock), 0,
+                       reinterpret_cast<hipStream_t>(stream), fine, batch, h, w, value_mask, dw,
+                       bdw, reinterpret_cast<const short *>(w1), b1,
+                       reinterpret_cast<const short *>(w2), b2, y, pred);
+    return xen::check(hipGetLastError(), "predict_kernel launch");
 }
 
 long long xen_nca_partial_rows(long long batch, int h, int w) {

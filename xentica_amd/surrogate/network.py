@@ -18,7 +18,7 @@ import torch
 from xentica_amd.core.exceptions import XenticaException
 from xentica_amd.runtime import hip
 
-__all__ = ['NCAFunction', 'NCASurrogate', 'encode', 'forward_raw',
+__all__ = ['NCAFunction', 'NCASurrogate', 'encode', 'forward_raw', 'predict',
            'PARAM_SHAPES', 'PARAMS']
 
 CHANNELS = 16
@@ -37,6 +37,8 @@ _SIGNATURES = {
     'xen_nca_encode': ([_vp, _ll, _i, _i, _u32, _vp, _vp, _vp], _i),
     'xen_nca_forward': ([_vp, _ll, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                          _vp, _vp, _vp], _i),
+    'xen_nca_predict': ([_vp, _ll, _i, _i, _u32, _vp, _vp, _vp, _vp, _vp,
+                         _vp, _vp, _vp, _vp], _i),
     'xen_nca_partial_rows': ([_ll, _i, _i], _ll),
     'xen_nca_backward': ([_vp, _vp, _ll, _i, _i, _vp, _vp, _vp, _vp, _vp,
                           _vp, _vp, _ll, _vp, _vp], _i),
@@ -111,6 +113,32 @@ def forward_raw(x, params, y, pred=None, stream=None):
         _dev(y, torch.float32, "y"),
         _dev(pred, torch.float32, "pred") if pred is not None else None,
         _stream(stream)), "nca forward")
+
+
+def predict(fine_ptr, batch, h, w, params, y=None, pred=None, value_mask=0xFF,
+            stream=None):
+    """
+    :func:`encode` and :func:`forward_raw` in one launch, without ``x``:
+    the fine uint8 field at ``fine_ptr`` (``[batch, 16h, 16w]``) to ``y``
+    ``[batch, h, w, 16]`` and/or ``pred`` ``[batch, h, w]``, bit for bit
+    what the two launches give.  ``h`` must be a multiple of 8 and ``w``
+    of 16.
+    """
+    if y is None and pred is None:
+        raise XenticaException("predict needs y or pred")
+    if y is not None and tuple(y.shape) != (batch, h, w, CHANNELS):
+        raise XenticaException("y has the wrong shape")
+    if pred is not None and tuple(pred.shape) != (batch, h, w):
+        raise XenticaException("pred has the wrong shape")
+    dw, bdw, w1, b1, w2, b2 = params
+    hip.check(_lib().xen_nca_predict(
+        _vp(fine_ptr), batch, h, w, value_mask,
+        _dev(dw, torch.float32, "dw"), _dev(bdw, torch.float32, "bdw"),
+        _dev(w1, torch.bfloat16, "w1"), _dev(b1, torch.float32, "b1"),
+        _dev(w2, torch.bfloat16, "w2"), _dev(b2, torch.float32, "b2"),
+        _dev(y, torch.float32, "y") if y is not None else None,
+        _dev(pred, torch.float32, "pred") if pred is not None else None,
+        _stream(stream)), "nca predict")
 
 
 class NCAFunction(torch.autograd.Function):
