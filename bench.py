@@ -78,7 +78,9 @@ def autotune(field, stream, trial_steps=96):
     best = None
     for depth in (5, 6, 8):
         for waves in (2, 3, 4, 6, 8):
-            strips = max(1, min(field.height, -(-waves * simds // tiles_x)))
+            # rounded down: one wave more than waves * simds puts a whole
+            # extra workgroup on one CU, which then finishes last
+            strips = max(1, min(field.height, waves * simds // tiles_x))
             field.depth, field.strips = depth, strips
             timed(field, stream, 2 * depth)
             ms = timed(field, stream, trial_steps)
@@ -482,6 +484,9 @@ def main():
     parser.add_argument("--warmup", type=int, default=200)
     parser.add_argument("--depth", type=int, default=0,
                         help="temporal depth (0 = autotune)")
+    parser.add_argument("--strips", type=int, default=0,
+                        help="row strips with --depth (0 = the library's "
+                             "choice)")
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--no-plain", action="store_true")
     parser.add_argument("--no-batch", action="store_true")
@@ -513,6 +518,8 @@ def main():
     field.set_cells(cells)
     depth = args.depth or autotune(field, stream)
     field.depth = depth
+    if args.depth:
+        field.strips = args.strips
     field.set_cells(cells)
     log("depth %d; warmup %d steps" % (depth, args.warmup))
     if args.warmup:
