@@ -143,6 +143,78 @@ def plain_engine(steps=20):
                          "in HBM, one launch pair per step"}
 
 
+def life512(steps=1000, cpu=True):
+    """
+    Life B3/S23 512x512 (word 42, 50 %) through the drop-in API, where the
+    per-step host work of ``model.step()`` dominates: ``steps`` steps on
+    the default engine (fused, launches deferred until a launch is full),
+    the same with the reference's test idiom (``crc32`` of
+    ``cells.get()`` after every step), and the plain engine; the hashes
+    are checked against the oracle's, which is also timed on the host.
+    """
+    import binascii
+    from examples.game_of_life import GameOfLife
+    from examples.workloads import Life512Experiment
+    from xentica_amd.runtime import hip
+    n = 512 * 512
+
+    def timed_run(engine, hashed):
+        exp = type("Life512" + engine.title(), (Life512Experiment, ),
+                   {"engine": engine})
+        model = GameOfLife(exp)
+        # one step before the clock starts: kernels compile, load and
+        # launch for the first time, work buffers are allocated
+        model.step()
+        model.gpu.arrays.cells.get()
+        hip.device().synchronize()
+        hashes = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            model.step()
+            if hashed:
+                hashes.append(binascii.crc32(
+                    model.gpu.arrays.cells.get()[:model.cells_num]))
+        model.gpu.arrays.cells.get()
+        hip.device().synchronize()
+        return time.perf_counter() - t0, hashes
+
+    fused_s, _ = timed_run("fused", False)
+    hashed_s, hashes = timed_run("fused", True)
+    plain_s, _ = timed_run("plain", False)
+    out = {
+        "workload": "Life B3/S23 512x512 torus, word 42, RandInt(0, 1), "
+                    "%d steps through model.step() after one warm-up step"
+                    % steps,
+        "fused": {"value": n * steps / fused_s, "unit": "cell-updates/s",
+                  "us_per_step": fused_s / steps * 1e6},
+        "fused_crc_every_step": {"value": n * steps / hashed_s,
+                                 "unit": "cell-updates/s",
+                                 "us_per_step": hashed_s / steps * 1e6},
+        "plain": {"value": n * steps / plain_s, "unit": "cell-updates/s",
+                  "us_per_step": plain_s / steps * 1e6},
+    }
+    from oracle import oracle
+    field = oracle.Field((512, 512), 8, np.uint8,
+                         GameOfLife(Life512Experiment).gpu.arrays.cells.get()
+                         [:n])
+    cores = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    want = oracle.lifelike(field, oracle.moore_deltas(2), oracle.moore_rev(2),
+                           8, 12, steps + 1, threads=cores, hashes=True)
+    cpu_s = time.perf_counter() - t0
+    # the timed runs cover steps 2 .. steps + 1 (step 1 is the warm-up)
+    out["hashes_match_oracle"] = [int(h) for h in want[1:]] == hashes
+    out["crc_step_%d" % (steps + 1)] = hashes[-1]
+    if cpu:
+        out["cpu_baseline"] = {
+            "value": n * (steps + 1) / cpu_s, "unit": "cell-updates/s",
+            "cores": cores, "kind": "port",
+            "sample": "the whole workload: %d steps with colours and a CRC "
+                      "of every step, reference plain structure, C + OpenMP"
+                      % (steps + 1)}
+    return out
+
+
 def batched(steps=1000, plain_steps=10):
     """
     The batch workload: 256 Life-like tori, sides 64-2048, own rules.
@@ -469,6 +541,7 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
 
 
 _LEGS = {
+    "life512": life512,
     "life3d": life3d,
     "energy": energy_life,
     "batch": batched,
@@ -503,7 +576,7 @@ def main():
     if args.only:
         log("%s only ..." % args.only)
         kwargs = ({"cpu": not args.no_cpu}
-                  if args.only in ("life3d", "energy") else {})
+                  if args.only in ("life512", "life3d", "energy") else {})
         print(json.dumps({args.only: _LEGS[args.only](**kwargs)}), flush=True)
         return 0
 
@@ -567,6 +640,8 @@ def main():
         log("plain engine ...")
         result["plain_engine"] = plain_engine()
         result["speedup_vs_plain"] = value / result["plain_engine"]["value"]
+    log("Life 512^2 through model.step() ...")
+    result["life512"] = life512(cpu=not args.no_cpu)
     if not args.no_3d:
         log("3-D workload ...")
         result["life3d"] = life3d(cpu=not args.no_cpu)

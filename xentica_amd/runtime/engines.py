@@ -50,6 +50,8 @@ class PlainEngine:
         if model.cells_num >= 2 ** 32:
             raise XenticaException(
                 "the plain engine indexes cells with 32-bit unsigned ints")
+        self._key = None
+        self._emit = self._absorb = None
 
     def launch_elementwise(self, kernel, args, count):
         """Launch a grid-stride kernel over ``count`` items."""
@@ -57,20 +59,33 @@ class PlainEngine:
                block=(BLOCK, 1, 1))
 
     def step(self):
-        """Launch ``emit`` then ``absorb`` with current parameter values."""
+        """Launch ``emit`` then ``absorb`` with current parameter values.
+
+        The two launches are prepared once (arguments converted, pointer
+        arrays built) and reused while the parameter values and the device
+        arrays stay the same: at small fields the per-step host work is
+        most of a step.
+        """
         model = self.model
-        gpu = model.gpu
-        kernels = gpu.kernels
-        cells_num = np.int64(model.cells_num)
-        args = [param.dtype(param.value) for param in model.emit_params]
-        self.launch_elementwise(
-            kernels.emit, [gpu.arrays.cells] + args + [cells_num],
-            model.cells_num)
-        args = [param.dtype(param.value) for param in model.absorb_params]
-        self.launch_elementwise(
-            kernels.absorb,
-            [gpu.arrays.cells, gpu.arrays.colors] + args + [cells_num],
-            model.cells_num)
+        arrays = model.gpu.arrays
+        key = (tuple(param.value for param in model.emit_params),
+               tuple(param.value for param in model.absorb_params),
+               arrays.cells.device_ptr(), arrays.colors.device_ptr())
+        if key != self._key:
+            kernels = model.gpu.kernels
+            cells_num = np.int64(model.cells_num)
+            grid = (elementwise_grid(model.cells_num), 1, 1)
+            args = [param.dtype(param.value) for param in model.emit_params]
+            self._emit = kernels.emit.prepare(
+                arrays.cells, *args, cells_num, grid=grid, block=(BLOCK, 1, 1))
+            args = [param.dtype(param.value)
+                    for param in model.absorb_params]
+            self._absorb = kernels.absorb.prepare(
+                arrays.cells, arrays.colors, *args, cells_num, grid=grid,
+                block=(BLOCK, 1, 1))
+            self._key = key
+        self._emit()
+        self._absorb()
 
     def flush(self):
         """Nothing is deferred: launches are already queued in order."""

@@ -659,6 +659,7 @@ class FusedEngine:
         self._pending_values = None
         self._other = None
         self._in_other = False
+        self._launches = {}
         arrays = model.gpu.arrays
         arrays.cells.before_read = self.flush
         arrays.colors.before_read = self.flush
@@ -693,15 +694,24 @@ class FusedEngine:
         arrays = self.model.gpu.arrays
         plane0 = arrays.cells.device_ptr()
         colors = arrays.colors.device_ptr()
-        grid = _persistent_grid(self._inst.n_tiles, self.program.lds_bytes)
         src, dst = ((self._other.ptr, plane0) if self._in_other
                     else (plane0, self._other.ptr))
-        self.program.function(
-            np.uint64(src), np.uint64(dst), np.uint64(colors),
-            np.uint64(self._inst.table_buf.ptr),
-            np.uint64(self._inst.tiles_buf.ptr),
-            np.int32(self._inst.n_tiles), np.int32(steps),
-            grid=(grid, 1, 1), block=(BLOCK, 1, 1))
+        # launches prepared once per (direction, step count, arrays)
+        key = (src, dst, colors, steps)
+        launch = self._launches.get(key)
+        if launch is None:
+            grid = _persistent_grid(self._inst.n_tiles,
+                                    self.program.lds_bytes)
+            launch = self.program.function.prepare(
+                np.uint64(src), np.uint64(dst), np.uint64(colors),
+                np.uint64(self._inst.table_buf.ptr),
+                np.uint64(self._inst.tiles_buf.ptr),
+                np.int32(self._inst.n_tiles), np.int32(steps),
+                grid=(grid, 1, 1), block=(BLOCK, 1, 1))
+            if len(self._launches) > 64:
+                self._launches.clear()
+            self._launches[key] = launch
+        launch()
         self._in_other = not self._in_other
 
     def flush(self):

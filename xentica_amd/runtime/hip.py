@@ -265,6 +265,35 @@ class Function:
             "launching %s" % self.name)
 
 
+    def prepare(self, *args, grid=(1, 1, 1), block=(256, 1, 1), shared=0,
+                stream=None):
+        """A launch whose arguments are converted once; call it to launch
+        (for launches repeated with the same arguments, e.g. every step)."""
+        return PreparedLaunch(self, args, grid, block, shared, stream)
+
+
+class PreparedLaunch:
+    """A kernel launch with its argument array built in advance."""
+
+    def __init__(self, function, args, grid, block, shared, stream):
+        self.function = function  # keeps the module loaded
+        self._cargs = [_as_ctype(arg) for arg in args]
+        self._ptrs = (_vp * max(1, len(self._cargs)))(
+            *[ctypes.cast(ctypes.pointer(c), _vp) for c in self._cargs])
+        grid = tuple(grid) + (1,) * (3 - len(grid))
+        block = tuple(block) + (1,) * (3 - len(block))
+        self._stream = stream  # keeps the stream alive
+        self._call = (function.handle, grid[0], grid[1], grid[2], block[0],
+                      block[1], block[2], shared,
+                      stream.handle if stream else None, self._ptrs)
+        self._what = "launching %s" % function.name
+
+    def __call__(self):
+        status = library().xen_launch(*self._call)
+        if status:
+            check(status, self._what)
+
+
 class Module:
     """A loaded code object."""
 
