@@ -215,6 +215,56 @@ def life512(steps=1000, cpu=True):
     return out
 
 
+def render_leg(steps=8, frames=5, launches=20):
+    """
+    ``render()`` of a 4096x4096 Life field (viewport = field, zoom 1) after
+    ``steps`` fused steps: the whole call (kernel, device-to-host copy of the
+    int32 image, the reference's ``astype(uint8)``) and the render kernel
+    alone; the image is checked against the plain engine's.
+    """
+    from examples.game_of_life import GameOfLife
+    from examples.workloads import Life512Experiment
+    from xentica_amd.runtime import hip
+    size = (4096, 4096)
+    models = {}
+    for engine in ("fused", "plain"):
+        exp = type("Render4k" + engine.title(), (Life512Experiment, ),
+                   {"size": size, "engine": engine})
+        model = GameOfLife(exp)
+        model.set_viewport(size)
+        for _ in range(steps):
+            model.step()
+        models[engine] = model
+    model = models["fused"]
+    image = model.render()
+    same = bool(np.array_equal(image, models["plain"].render()))
+    hip.device().synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        model.render()
+    call_ms = (time.perf_counter() - t0) / frames * 1e3
+    pixels = size[0] * size[1]
+    args = (model.renderer.get_args_vals(model)
+            + [param.dtype(param.value) for param in model._render_params]
+            + [np.int64(pixels)])
+    kernel = model.gpu.kernels.render
+    hip.device().synchronize()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        model._engine.launch_elementwise(kernel, args, pixels)
+    hip.device().synchronize()
+    kernel_us = (time.perf_counter() - t0) / launches * 1e6
+    return {
+        "workload": "render() of Life 4096x4096 after %d fused steps, "
+                    "viewport 4096x4096, zoom 1" % steps,
+        "frame_ms": call_ms, "frames_per_s": 1e3 / call_ms,
+        "kernel_us": kernel_us,
+        "kernel_GBps": pixels * 24 / (kernel_us * 1e-6) / 1e9,
+        "kernel_bytes_per_pixel": 24,
+        "image_matches_plain_engine": same,
+    }
+
+
 def batched(steps=1000, plain_steps=10):
     """
     The batch workload: 256 Life-like tori, sides 64-2048, own rules.
@@ -542,6 +592,7 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
 
 _LEGS = {
     "life512": life512,
+    "render": render_leg,
     "life3d": life3d,
     "energy": energy_life,
     "batch": batched,
@@ -642,6 +693,8 @@ def main():
         result["speedup_vs_plain"] = value / result["plain_engine"]["value"]
     log("Life 512^2 through model.step() ...")
     result["life512"] = life512(cpu=not args.no_cpu)
+    log("render ...")
+    result["render"] = render_leg()
     if not args.no_3d:
         log("3-D workload ...")
         result["life3d"] = life3d(cpu=not args.no_cpu)

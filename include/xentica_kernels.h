@@ -70,6 +70,12 @@ int xen_life3d_u8_direct_step(const void *src, void *dst, int width, int height,
 int xen_bits_pack(const uint8_t *cells, uint32_t *bits, long long n_cells, xen_stream stream);
 int xen_bits_unpack(const uint32_t *bits, uint8_t *cells, long long n_cells, xen_stream stream);
 
+/* The low byte of each of n int32 values into dst: the
+ * `img.get().astype(numpy.uint8)` of the reference's render()
+ * (xentica/core/base.py:753) done on the device, so that a quarter of the
+ * bytes cross to the host.  src 16-byte aligned, dst 4-byte aligned. */
+int xen_low_bytes(const int32_t *src, uint8_t *dst, long long n, xen_stream stream);
+
 /* ---- device CRC-32 (the binascii.crc32 of tests/core/test_base.py:39-41) ----
  * Zero-initialised, un-inverted CRC-32 register of every `chunk`-byte
  * chunk of a byte plane (the last chunk may be shorter) into

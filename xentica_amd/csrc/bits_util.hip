@@ -101,6 +101,22 @@ __global__ __launch_bounds__(kBlock) void crc_bits_kernel(const uint32_t *__rest
     partials[c] = crc;
 }
 
+// The low byte of every int32 value (what C's and NumPy's int32 -> uint8
+// casts give), four values per thread: one 16-byte load, one 4-byte store.
+__global__ __launch_bounds__(kBlock) void low_bytes_kernel(const int32_t *__restrict__ src,
+                                                           uint8_t *__restrict__ dst,
+                                                           long long n) {
+    const long long i = 4 * (blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x);
+    if (i + 3 < n) {
+        const int4 v = *reinterpret_cast<const int4 *>(src + i);
+        *reinterpret_cast<uint32_t *>(dst + i) =
+            (static_cast<uint32_t>(v.x) & 0xFFu) | ((static_cast<uint32_t>(v.y) & 0xFFu) << 8) |
+            ((static_cast<uint32_t>(v.z) & 0xFFu) << 16) | (static_cast<uint32_t>(v.w) << 24);
+    } else {
+        for (long long k = i; k < n; ++k) dst[k] = static_cast<uint8_t>(src[k]);
+    }
+}
+
 unsigned groups_for(long long items) {
     return static_cast<unsigned>((items + kBlock - 1) / kBlock);
 }
@@ -125,6 +141,16 @@ int xen_bits_unpack(const uint32_t *bits, uint8_t *cells, long long n_cells, xen
     hipLaunchKernelGGL(unpack_kernel, dim3(groups_for(words)), dim3(kBlock), 0,
                        reinterpret_cast<hipStream_t>(stream), bits, cells, words);
     return xen::check(hipGetLastError(), "unpack_kernel launch");
+}
+
+int xen_low_bytes(const int32_t *src, uint8_t *dst, long long n, xen_stream stream) {
+    if (n < 0) return xen::fail("xen_low_bytes: negative count", 22);
+    if (!n) return 0;
+    if ((reinterpret_cast<uintptr_t>(src) & 15) || (reinterpret_cast<uintptr_t>(dst) & 3))
+        return xen::fail("xen_low_bytes: src must be 16-byte and dst 4-byte aligned", 22);
+    hipLaunchKernelGGL(low_bytes_kernel, dim3(groups_for((n + 3) / 4)), dim3(kBlock), 0,
+                       reinterpret_cast<hipStream_t>(stream), src, dst, n);
+    return xen::check(hipGetLastError(), "low_bytes_kernel launch");
 }
 
 int xen_crc32_u8_partials(const uint8_t *data, long long n, long long chunk, uint32_t *partials,

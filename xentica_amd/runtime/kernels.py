@@ -14,7 +14,7 @@ from xentica_amd.runtime.crc import combine_partials
 __all__ = [
     'life_bits_launch', 'life_bits_run', 'life3d_bits_step',
     'life3d_u8_step', 'life3d_u8_direct_step', 'bits_pack', 'bits_unpack',
-    'crc32_u8', 'crc32_bits', 'CRC_CHUNK',
+    'crc32_u8', 'crc32_bits', 'CRC_CHUNK', 'low_bytes',
 ]
 
 _vp = ctypes.c_void_p
@@ -31,6 +31,7 @@ _SIGNATURES = {
     'xen_life3d_u8_direct_step': [_vp, _vp, _i, _i, _i, _u32, _u32, _vp],
     'xen_bits_pack': [_vp, _vp, _ll, _vp],
     'xen_bits_unpack': [_vp, _vp, _ll, _vp],
+    'xen_low_bytes': [_vp, _vp, _ll, _vp],
     'xen_crc32_u8_partials': [_vp, _ll, _ll, _vp, _vp],
     'xen_crc32_bits_partials': [_vp, _ll, _ll, _vp, _vp],
 }
@@ -109,6 +110,12 @@ def bits_unpack(bits, cells, n_cells, stream=None):
 
 def _partials_buffer(n_chunks):
     return hip.Buffer(max(1, n_chunks) * 4)
+
+
+def low_bytes(src_ptr, dst_ptr, count, stream=None):
+    """``dst[i] = uint8(src[i])`` for ``count`` int32 values on the device."""
+    hip.check(_lib().xen_low_bytes(src_ptr, dst_ptr, count, _s(stream)),
+              "xen_low_bytes")
 
 
 def crc32_u8(data_ptr, nbytes, stream=None, chunk=CRC_CHUNK):
