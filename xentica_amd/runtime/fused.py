@@ -78,13 +78,22 @@ def eligible(model):
 class FusedProgram:
     """The generated fused kernel of one model class and tile shape."""
 
-    #: (tile, depth) candidates, best first; the first that fits LDS wins.
-    CANDIDATES = (((32, 32), 4), ((64, 16), 4), ((32, 16), 4), ((32, 32), 2),
-                  ((32, 16), 2), ((16, 16), 2), ((16, 16), 1), ((8, 8), 1))
+    #: (tile, depth) candidates, best first, for states of at most four
+    #: bytes and for wider ones; the first that fits the LDS budget wins.
+    #: The orders follow what the autotuner measured on MI355X: byte-sized
+    #: Life-like states run fastest on 64x16 tiles at 6-8 steps per launch,
+    #: 16-byte words (EnergyLife) on 32x16 or 16x16 tiles at 2-3.
+    CANDIDATES = (((64, 16), 8), ((64, 16), 6), ((32, 32), 4), ((32, 16), 4),
+                  ((32, 32), 2), ((32, 16), 2), ((16, 16), 2), ((16, 16), 1),
+                  ((8, 8), 1))
+    CANDIDATES_WIDE = (((32, 16), 3), ((16, 16), 2), ((32, 16), 2),
+                       ((16, 16), 1), ((8, 8), 1))
 
     def __init__(self, model, tile=None, depth=None, fold=True, split=True):
         if tile is None or depth is None:
-            for cand_tile, cand_depth in self.CANDIDATES:
+            wide = np.dtype(model.main.dtype).itemsize > 4
+            for cand_tile, cand_depth in (self.CANDIDATES_WIDE if wide
+                                          else self.CANDIDATES):
                 try:
                     self.__init__(model, cand_tile, cand_depth, fold, split)
                     return
