@@ -100,7 +100,7 @@ def traffic_per_launch():
         return json.load(summary).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(cells, steps=4):
+def cpu_baseline(cells, steps=20):
     """The oracle on a bounded sample: the same field, ``steps`` steps."""
     from oracle import oracle
     cores = len(os.sched_getaffinity(0))
@@ -357,7 +357,7 @@ def life3d(steps=200, cpu=True):
     return out
 
 
-def life3d_cpu_baseline(born, sustain, side=256, steps=10):
+def life3d_cpu_baseline(born, sustain, side=256, steps=40):
     """
     The oracle on a 256^3 sample of the same workload (the reference's
     plain layout of 1024^3 would need 27 GiB of buffer planes): same rule,
@@ -381,7 +381,7 @@ def life3d_cpu_baseline(born, sustain, side=256, steps=10):
                       "planes, no colours, C + OpenMP" % (side, steps)}
 
 
-def energy_cpu_baseline(steps=20):
+def energy_cpu_baseline(steps=50):
     """The oracle on the same 4096^2 EnergyLife field, ``steps`` steps
     (colours included, as the fused engine computes them)."""
     from oracle import oracle
