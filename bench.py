@@ -334,7 +334,8 @@ def life3d(steps=200, cpu=True):
                        "1024^3 torus, 10 %% density, %d steps" % steps}
     legs = (("bitpacked", Life3DField, {}, 0.25, steps),
             ("uint8", Life3DBytes, {}, 2.0, steps),
-            ("uint8_direct", Life3DBytes, {"direct": True}, 2.0, 20))
+            ("uint8_direct", Life3DBytes, {"direct": True}, 2.0,
+             min(20, steps)))
     for name, cls, kwargs, bpc, count in legs:
         field = cls(size, born, sustain, stream=stream, **kwargs)
         field.set_cells(cells)
@@ -617,6 +618,9 @@ def main():
     parser.add_argument("--no-3d", action="store_true")
     parser.add_argument("--no-energy", action="store_true")
     parser.add_argument("--no-surrogate", action="store_true")
+    parser.add_argument("--leg-steps", type=int, default=0,
+                        help="steps of the workload with --only life3d, "
+                             "energy or life512 (0 = the leg's default)")
     parser.add_argument("--only", choices=sorted(_LEGS),
                         help="run one secondary workload alone (profiling)")
     args = parser.parse_args()
@@ -628,6 +632,8 @@ def main():
         log("%s only ..." % args.only)
         kwargs = ({"cpu": not args.no_cpu}
                   if args.only in ("life512", "life3d", "energy") else {})
+        if args.leg_steps and args.only in ("life512", "life3d", "energy"):
+            kwargs["steps"] = args.leg_steps
         print(json.dumps({args.only: _LEGS[args.only](**kwargs)}), flush=True)
         return 0
 
