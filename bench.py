@@ -265,7 +265,7 @@ def render_leg(steps=8, frames=5, launches=20):
     }
 
 
-def batched(steps=1000, plain_steps=10):
+def batched(steps=1000, plain_steps=10, tiles=None, depths=None):
     """
     The batch workload: 256 Life-like tori, sides 64-2048, own rules.
 
@@ -281,7 +281,7 @@ def batched(steps=1000, plain_steps=10):
     exps = batch_experiments()
     models = [LifeLikeRule(exp) for exp in exps]
     total = sum(model.cells_num for model in models)
-    batch, tuned = autotune(models, log=log)
+    batch, tuned = autotune(models, log=log, **_tune_space(tiles, depths))
     batch.run(steps)                       # capture, warm up
     batch.synchronize()
     start, stop = hip.Event(), hip.Event()
@@ -291,21 +291,24 @@ def batched(steps=1000, plain_steps=10):
     stop.synchronize()
     ms = start.elapsed_ms(stop)
     value = total * steps / (ms / 1e3)
-    plain_models = []
-    for exp in exps:
-        plain_exp = type(exp.__name__ + "Plain", (exp,), {"engine": "plain"})
-        plain_models.append(LifeLikeRule(plain_exp))
-    for model, exp in zip(plain_models, exps):
-        model.rule = exp.rule
-        model.step()
-    hip.device().synchronize()
-    t0 = time.perf_counter()
-    for _ in range(plain_steps):
+    plain_value = None
+    if plain_steps:
+        plain_models = []
+        for exp in exps:
+            plain_exp = type(exp.__name__ + "Plain", (exp,),
+                             {"engine": "plain"})
+            plain_models.append(LifeLikeRule(plain_exp))
         for model, exp in zip(plain_models, exps):
             model.rule = exp.rule
             model.step()
-    hip.device().synchronize()
-    plain_value = total * plain_steps / (time.perf_counter() - t0)
+        hip.device().synchronize()
+        t0 = time.perf_counter()
+        for _ in range(plain_steps):
+            for model, exp in zip(plain_models, exps):
+                model.rule = exp.rule
+                model.step()
+        hip.device().synchronize()
+        plain_value = total * plain_steps / (time.perf_counter() - t0)
     return {
         "workload": "256 Life-like tori, sides log-uniform 64-2048, random "
                     "9-bit birth/survival masks, density 0.1-0.6, %d steps"
@@ -316,7 +319,7 @@ def batched(steps=1000, plain_steps=10):
         "autotune": {"%dx%d/d%d" % (t[0], t[1], d): v
                      for (t, d), v in tuned.items()},
         "plain_loop_value": plain_value, "plain_loop_steps": plain_steps,
-        "speedup_vs_plain": value / plain_value,
+        "speedup_vs_plain": value / plain_value if plain_value else None,
     }
 
 
@@ -399,7 +402,8 @@ def energy_cpu_baseline(steps=50):
                       "planes of 128-bit words, colours), C + OpenMP" % steps}
 
 
-def energy_life(steps=5000, plain_steps=20, cpu=True):
+def energy_life(steps=5000, plain_steps=20, cpu=True, tiles=None,
+                depths=None):
     """
     EnergyLife on a 4096x4096 torus: 128-bit words (energy 16 bits, genome
     18, mode 1, xorshift32 32), word 42, ``steps`` steps by the fused
@@ -413,7 +417,7 @@ def energy_life(steps=5000, plain_steps=20, cpu=True):
     model = EnergyLife(EnergyLifeExperiment)
     n = model.cells_num
     total0 = int(energy_fields(model.gpu.arrays.cells.get()[:n])[0].sum())
-    batch, _ = autotune([model], log=log)
+    batch, _ = autotune([model], log=log, **_tune_space(tiles, depths))
     batch.run(10)
     batch.synchronize()
     start, stop = hip.Event(), hip.Event()
@@ -425,16 +429,18 @@ def energy_life(steps=5000, plain_steps=20, cpu=True):
     batch.scatter()
     energy, _, mode, _ = energy_fields(model.gpu.arrays.cells.get()[:n])
     value = n * steps / (ms / 1e3)
-    plain_exp = type("EnergyLifePlain", (EnergyLifeExperiment, ),
-                     {"engine": "plain"})
-    plain = EnergyLife(plain_exp)
-    plain.step()
-    hip.device().synchronize()
-    t0 = time.perf_counter()
-    for _ in range(plain_steps):
+    plain_value = None
+    if plain_steps:
+        plain_exp = type("EnergyLifePlain", (EnergyLifeExperiment, ),
+                         {"engine": "plain"})
+        plain = EnergyLife(plain_exp)
         plain.step()
-    hip.device().synchronize()
-    plain_value = n * plain_steps / (time.perf_counter() - t0)
+        hip.device().synchronize()
+        t0 = time.perf_counter()
+        for _ in range(plain_steps):
+            plain.step()
+        hip.device().synchronize()
+        plain_value = n * plain_steps / (time.perf_counter() - t0)
     bpc = 32.0
     return {
         "workload": "EnergyLife 4096x4096 torus, 128-bit words (energy 16, "
@@ -450,9 +456,19 @@ def energy_life(steps=5000, plain_steps=20, cpu=True):
         "energy_conserved": int(energy.sum()) == total0,
         "alive_fraction_end": float(mode.mean()),
         "plain_loop_value": plain_value, "plain_loop_steps": plain_steps,
-        "speedup_vs_plain": value / plain_value,
+        "speedup_vs_plain": value / plain_value if plain_value else None,
         **({"cpu_baseline": energy_cpu_baseline()} if cpu else {}),
     }
+
+
+def _tune_space(tiles, depths):
+    """Keyword arguments of ``autotune`` restricting its candidates."""
+    space = {}
+    if tiles:
+        space["tiles"] = tiles
+    if depths:
+        space["depths"] = depths
+    return space
 
 
 def _event_ms(stream, work, repeats):
@@ -620,7 +636,14 @@ def main():
     parser.add_argument("--no-surrogate", action="store_true")
     parser.add_argument("--leg-steps", type=int, default=0,
                         help="steps of the workload with --only life3d, "
-                             "energy or life512 (0 = the leg's default)")
+                             "energy, batch or life512 (0 = the leg's "
+                             "default)")
+    parser.add_argument("--fused-tile", default="",
+                        help="WxH: the fused legs (batch, energy) use this "
+                             "tile instead of autotuning it")
+    parser.add_argument("--fused-depth", type=int, default=0,
+                        help="the fused legs use this depth instead of "
+                             "autotuning it")
     parser.add_argument("--only", choices=sorted(_LEGS),
                         help="run one secondary workload alone (profiling)")
     args = parser.parse_args()
@@ -632,8 +655,17 @@ def main():
         log("%s only ..." % args.only)
         kwargs = ({"cpu": not args.no_cpu}
                   if args.only in ("life512", "life3d", "energy") else {})
-        if args.leg_steps and args.only in ("life512", "life3d", "energy"):
+        if args.leg_steps and args.only in ("life512", "life3d", "energy",
+                                            "batch"):
             kwargs["steps"] = args.leg_steps
+        if args.only in ("energy", "batch"):
+            if args.fused_tile:
+                kwargs["tiles"] = (tuple(int(v) for v in
+                                         args.fused_tile.split("x")), )
+            if args.fused_depth:
+                kwargs["depths"] = (args.fused_depth, )
+            if args.no_plain:
+                kwargs["plain_steps"] = 0
         print(json.dumps({args.only: _LEGS[args.only](**kwargs)}), flush=True)
         return 0
 
