@@ -489,7 +489,9 @@ class CellularAutomaton(Translator, metaclass=BSCA):
 
     #: Step engine: ``"auto"`` (the fused LDS-tile engine when the model
     #: qualifies, else ``"plain"``), ``"fused"``, ``"plain"`` (the
-    #: reference's structure); an experiment may override it.
+    #: reference's structure), or ``"fused_generic"``/``"fused_unfolded"``
+    #: (other forms of the fused kernel, for checking); an experiment may
+    #: override it.
     engine = "auto"
     #: Floating-point contraction of the generated code: ``"off"`` keeps
     #: every ``a * b + c`` as two roundings, ``"fast"`` lets the compiler
