@@ -99,13 +99,15 @@ def kernel(src, W, H, tiles_x, tiles_y, K, born, sustain):
             feed(i, load_row(i), False)
         i = 2 * K
         a, b, c = load_row(i), load_row(i + 1), load_row(i + 2)
+        na, nb, nc = load_row(i + 3), load_row(i + 4), load_row(i + 5)
         iters = 0
         while i + 2 < L:
-            na, nb, nc = load_row(i + 3), load_row(i + 4), load_row(i + 5)
+            fa, fb, fc = load_row(i + 6), load_row(i + 7), load_row(i + 8)
             for q, v in enumerate((a, b, c)):
                 feed(i + q, v, True)
                 store_row(lev[K][(i + q - K) % 3][0], i + q - 2 * K)
             a, b, c = na, nb, nc
+            na, nb, nc = fa, fb, fc
             i += 3; iters += 1
             assert iters < 100000
         if i < L:

@@ -252,12 +252,16 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
         }
     }
     // steady state: rows 2K .. L-1, three per iteration so that the
-    // window slots are compile-time constants
+    // window slots are compile-time constants; input rows are loaded two
+    // iterations (six feeds) ahead of their use, which covers the load
+    // latency with fewer waves per SIMD (rows past L-1 are wrapped rows,
+    // in bounds and unused)
     constexpr int P0 = (2 * K) % 3, P1 = (2 * K + 1) % 3, P2 = (2 * K + 2) % 3;
     int i = 2 * K;
     uint32_t a = load_row(i), b = load_row(i + 1), c = load_row(i + 2);
+    uint32_t na = load_row(i + 3), nb = load_row(i + 4), nc = load_row(i + 5);
     for (; i + 2 < L; i += 3) {
-        const uint32_t na = load_row(i + 3), nb = load_row(i + 4), nc = load_row(i + 5);
+        const uint32_t fa = load_row(i + 6), fb = load_row(i + 7), fc = load_row(i + 8);
         feed<K, kFast, P0, true>(w, a, i, rule);
         store_row(level_k_row<K, P0>(w), i - 2 * K);
         feed<K, kFast, P1, true>(w, b, i + 1, rule);
@@ -267,6 +271,9 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
         a = na;
         b = nb;
         c = nc;
+        na = fa;
+        nb = fb;
+        nc = fc;
     }
     if (i < L) {
         feed<K, kFast, P0, true>(w, a, i, rule);
