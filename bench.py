@@ -634,6 +634,8 @@ def main():
     parser.add_argument("--no-3d", action="store_true")
     parser.add_argument("--no-energy", action="store_true")
     parser.add_argument("--no-surrogate", action="store_true")
+    parser.add_argument("--no-512", action="store_true")
+    parser.add_argument("--no-render", action="store_true")
     parser.add_argument("--leg-steps", type=int, default=0,
                         help="steps of the workload with --only life3d, "
                              "energy, batch or life512 (0 = the leg's "
@@ -729,10 +731,12 @@ def main():
         log("plain engine ...")
         result["plain_engine"] = plain_engine()
         result["speedup_vs_plain"] = value / result["plain_engine"]["value"]
-    log("Life 512^2 through model.step() ...")
-    result["life512"] = life512(cpu=not args.no_cpu)
-    log("render ...")
-    result["render"] = render_leg()
+    if not args.no_512:
+        log("Life 512^2 through model.step() ...")
+        result["life512"] = life512(cpu=not args.no_cpu)
+    if not args.no_render:
+        log("render ...")
+        result["render"] = render_leg()
     if not args.no_3d:
         log("3-D workload ...")
         result["life3d"] = life3d(cpu=not args.no_cpu)
