@@ -179,6 +179,33 @@ def test_static_border_split_loop_every_tile(tile):
                                   plain.gpu.arrays.colors.get())
 
 
+def test_batch_rule_change_between_graph_replays():
+    """Per-instance parameter values changed between runs of a captured
+    graph take effect (the table is re-uploaded, the graph reads it)."""
+    exps = batch_experiments(count=4, seed=9, side_min=30, side_max=120)
+    models = [LifeLikeRule(exp) for exp in exps]
+    batch = Batch(models, tile=(64, 16), depth=3)
+    plain = [LifeLikeRule(type("P" + exp.__name__, (exp, ), {"engine": "plain"}))
+             for exp in exps]
+    rules = [exp.rule for exp in exps]
+    schedule = [None, 6152, None, 0b111000 | (0b1100 << 9)]
+    for new_rule in schedule:
+        if new_rule is not None:
+            models[1].rule = new_rule
+            rules[1] = new_rule
+        batch.run(7, graph=True)
+        # a parameter is class-wide, as in the reference: a single model
+        # steps with the value last set on the class
+        for model, rule in zip(plain, rules):
+            model.rule = rule
+            for _ in range(7):
+                model.step()
+    batch.synchronize()
+    batch.scatter()
+    for a, b in zip(models, plain):
+        assert crc(a) == crc(b)
+
+
 def test_batch_graph_replays():
     """The captured graph replayed several times equals eager launches."""
     exps = batch_experiments(count=6, seed=3, side_min=40, side_max=200)
