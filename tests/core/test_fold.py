@@ -95,6 +95,13 @@ def test_color_guard_only_for_pure_colors():
     from examples.noisetv import NoiseTV, NoiseTVExperiment
     life = FusedProgram(GameOfLife(GOLExperiment))
     assert life.color_guard and "if (_inner) {" in life.source
+    # the last-step form holds each step with and without the colour and
+    # never reads the stored colours or the clamp's old colour
+    assert "#define _CFAST 1" in life.source_fast
+    assert "if (_last) {" in life.source_fast
+    assert "old_col" not in life.source_fast and "old_col" in life.source
     # NoiseTV's colour advances the generator: it must run on every cell
+    # and every step
     noise = FusedProgram(NoiseTV(NoiseTVExperiment))
-    assert not noise.color_guard and "if (_inner) {" not in noise.source
+    assert not noise.color_guard and "if (_inner" not in noise.source
+    assert noise.color_plan is None and noise.source_fast is None

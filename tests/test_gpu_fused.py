@@ -11,16 +11,27 @@ import pytest
 
 from examples.game_of_life import (
     GameOfLife, GameOfLifeStatic, GameOfLifeColor, GOLExperiment,
-    GOLExperimentColor, LifelikeCA, DiamoebaExperiment)
+    GOLExperiment2, GOLExperimentColor, LifelikeCA, DiamoebaExperiment)
 from examples.shifting_sands import ShiftingSands, ShiftingSandsExperiment
 from examples.noisetv import NoiseTV, NoiseTVExperiment
 from examples.evolife import (
     EvoLife, CrossbreedingExperiment, CrossbreedingExperiment2)
 from examples.workloads import (
     Life512Experiment, LifeLikeRule, batch_experiments)
+from xentica_amd.core import color_effects
 from xentica_amd.runtime.fused import Batch, FusedEngine, TUNE_TILES
 
 pytestmark = pytest.mark.gpu
+
+
+class PlainColorLife(GameOfLife):
+    """Life whose colour is stored as computed (no moving average)."""
+
+    @color_effects.ColorEffect
+    def color(self):
+        """Live cells white, dead ones dark blue."""
+        val = self.main.state * 255
+        return (val, val, 40 + val)
 
 
 def fused(exp, engine="fused"):
@@ -88,6 +99,85 @@ def test_colors_match_plain_every_step():
         assert crc(fast) == crc(plain), step
         assert np.array_equal(fast.gpu.arrays.colors.get(),
                               plain.gpu.arrays.colors.get()), step
+
+
+@pytest.mark.parametrize("engine", ["fused", "fused_generic"])
+@pytest.mark.parametrize("outside", [False, True])
+def test_loaded_colours_and_the_last_step_form(engine, outside):
+    """
+    Life's colour is 255 * state with fades of 255: from stored colours in
+    [0, 255] every step's colour is its new value, so the engine launches
+    the form that computes the colour on a launch's last step only.
+    Loaded colours within [0, 255] keep that form; rows of colours far
+    outside it (and single cells just outside) make the engine compute
+    every step's colour.  Both match the plain engine after launches of 1
+    to 8 steps.
+    """
+    plain = GameOfLife(GOLExperiment2)
+    fast = GameOfLife(fused(GOLExperiment2, engine))
+    program = fast._engine.program
+    assert program.color_plan == ((0, 255), ) * 3
+    cells = plain.gpu.arrays.cells.get()
+    rng = np.random.default_rng(5)
+    colors = rng.integers(0, 256, plain.cells_num * 3).astype(np.int32)
+    if outside:
+        width = plain.size[0]
+        colors[3 * width * 3:40 * width * 3] = rng.integers(
+            -2000, 2000, 37 * width * 3)
+        far = 200 * width * 3
+        colors[far:far + 4] = (256, -1, 255, 0)
+        colors[far + 3 * width * 20 + 7] = 1 << 30
+    for model in (plain, fast):
+        model._engine.load_state(cells, colors)
+    assert program.admits(fast.gpu.arrays.colors.bounds) == (not outside)
+    for count in (1, 8, 5, 16, 3):
+        run(plain, count)
+        run(fast, count)
+        assert crc(fast) == crc(plain), count
+        assert np.array_equal(fast.gpu.arrays.colors.get(),
+                              plain.gpu.arrays.colors.get()), count
+        want = None if outside else program.color_values
+        assert fast.gpu.arrays.colors.bounds == want
+
+
+def test_batch_with_colours_outside_the_plan():
+    """One instance's loaded colours lie outside [0, 255]: the batch runs
+    the every-step form, and every instance matches its plain twin."""
+    exps = batch_experiments(count=4, seed=3, side_min=30, side_max=200)
+    models = [LifeLikeRule(exp) for exp in exps]
+    twins = [LifeLikeRule(fused(exp, "plain")) for exp in exps]
+    odd = models[2]
+    colors = np.random.default_rng(1).integers(
+        -999, 999, odd.cells_num * 3).astype(np.int32)
+    for model in (odd, twins[2]):
+        model._engine.load_state(model.gpu.arrays.cells.get(), colors)
+    batch = Batch(models, tile=(64, 16), depth=6)
+    assert not batch.program.admits(batch.color_bounds)
+    batch.run(20)
+    batch.synchronize()
+    batch.scatter()
+    for exp, model, twin in zip(exps, models, twins):
+        twin.rule = exp.rule
+        run(twin, 20)
+        assert crc(model) == crc(twin), exp.__name__
+        assert np.array_equal(model.gpu.arrays.colors.get(),
+                              twin.gpu.arrays.colors.get()), exp.__name__
+        assert model.gpu.arrays.colors.bounds is None
+
+
+def test_memoryless_colour_only_last_step():
+    """A plain colour effect never reads the stored colour: the fused
+    kernel computes it on a launch's last step and does not load it."""
+    plain = PlainColorLife(GOLExperiment2)
+    fast = PlainColorLife(fused(GOLExperiment2))
+    assert fast._engine.program.color_plan == ()
+    assert "#define _CFAST 1" in fast._engine.program.source_fast
+    for count in (1, 8, 13):
+        run(plain, count)
+        run(fast, count)
+        assert crc(fast) == crc(plain), count
+        assert np.array_equal(fast.gpu.arrays.colors.get(),
+                              plain.gpu.arrays.colors.get()), count
 
 
 def test_life512_deferred_vs_oracle():

@@ -444,6 +444,7 @@ class GPUArrays:
         """Hold the cells (reference layout) and colours."""
         self.img = None
         self.colors = gpuarray.to_gpu(init_colors)
+        self.colors.track_bounds(3)
         self.cells = gpuarray.to_gpu(init_cells)
 
     def init_img(self, num_cells):

@@ -33,6 +33,9 @@ class ColorEffect(BscaDetectorMixin):
         """Wrap the model's colour function."""
         self.func = func
         self.effect = ""
+        #: the statements of ``effect`` that read the stored colour (``""``:
+        #: the colour written never depends on the colour before it)
+        self.memory = ""
 
     def __call__(self):
         """Emit the colour computation into the current kernel."""
@@ -77,11 +80,13 @@ class MovingAverage(ColorEffect):
             clamp = ("new_{c} = max(min(new_{c}, old_col.{x} - FADE_OUT), "
                      "min(max(new_{c}, old_col.{x} - FADE_OUT), "
                      "old_col.{x} + FADE_IN));\n")
+        self.memory = ("int3 old_col = col[i];\n"
+                       + "".join(clamp.format(c=c, x=x)
+                                 for c, x in (("r", "x"), ("g", "y"),
+                                              ("b", "z"))))
         self.effect = (
             "new_r *= SMOOTH_FACTOR;\n"
             "new_g *= SMOOTH_FACTOR;\n"
             "new_b *= SMOOTH_FACTOR;\n"
-            "int3 old_col = col[i];\n"
-            + "".join(clamp.format(c=c, x=x)
-                      for c, x in (("r", "x"), ("g", "y"), ("b", "z"))))
+            + self.memory)
         return super().__call__()

@@ -8,6 +8,12 @@ An array stays on the host until a kernel first needs its device address,
 so models can be built, seeded and inspected on machines without a GPU.
 An owner may attach a ``before_read`` hook that runs ahead of every
 ``.get()``; the engines use it to finish deferred steps first.
+
+An array of interleaved channels (the colours: red, green, blue per cell)
+can keep per-channel value bounds: computed from the data at every
+``set()``, updated by the engines after their kernels write, ``None``
+when unknown.  Code that writes the array through its device address must
+set ``bounds`` to ``None``.
 """
 import numpy as np
 
@@ -26,6 +32,14 @@ class DeviceArray:
         self._host = host.copy()
         self._buffer = None
         self.before_read = None
+        self.channels = None
+        self.bounds = None
+
+    def track_bounds(self, channels):
+        """Keep per-channel ``(min, max)`` bounds of the contents."""
+        self.channels = channels
+        self.bounds = (None if self._host is None
+                       else _channel_bounds(self._host, channels))
 
     @property
     def size(self):
@@ -68,11 +82,22 @@ class DeviceArray:
             self._host = host.copy()
         else:
             self._buffer.upload(host)
+        if self.channels:
+            self.bounds = _channel_bounds(host, self.channels)
 
     def __repr__(self):
         where = "device" if self.on_device else "host"
         return "DeviceArray(shape=%s, dtype=%s, %s)" % (
             self.shape, self.dtype, where)
+
+
+def _channel_bounds(host, channels):
+    """``((min, max), ...)`` of each channel of interleaved ``host``."""
+    view = host.reshape(-1, channels)
+    if not len(view):
+        return None
+    return tuple((int(view[:, c].min()), int(view[:, c].max()))
+                 for c in range(channels))
 
 
 def to_gpu(host):

@@ -86,6 +86,8 @@ class PlainEngine:
             self._key = key
         self._emit()
         self._absorb()
+        # the colour bounds the fused engine relies on are not followed here
+        arrays.colors.bounds = None
 
     def flush(self):
         """Nothing is deferred: launches are already queued in order."""

@@ -20,6 +20,11 @@ one generated kernel:
   ``xentica/core/base.py:254-283, 461-487``);
 * the colours of the output cells live in LDS for the whole launch, so
   the per-step colour effect (``color_effects.py:106-130``) is exact;
+  where the colour a step writes provably does not depend on the colour
+  before it (``runtime/ranges.py``) and the stored colours' bounds, which
+  the host follows, meet the proof's precondition, a second form of the
+  kernel computes the colour on a launch's last step only and does not
+  read the stored colours;
 * the kernel is persistent over a list of (instance, tile) work items
   described by a per-instance table (offsets, sizes, parameter values),
   so one launch advances any number of independent automata of uneven
@@ -33,11 +38,12 @@ reference's zero-initialised buffer planes).
 """
 import numpy as np
 
+from xentica_amd.core.color_effects import ColorEffect
 from xentica_amd.core.exceptions import XenticaException
 from xentica_amd.core.topology.border import TorusBorder, StaticBorder
 from xentica_amd.core.topology.lattice import OrthogonalLattice
 from xentica_amd.core.topology.neighborhood import OrthogonalNeighborhood
-from xentica_amd.runtime import hip
+from xentica_amd.runtime import hip, ranges
 
 __all__ = ['FusedProgram', 'FusedEngine', 'Batch', 'eligible', 'autotune',
            'TUNE_TILES', 'TUNE_DEPTHS']
@@ -150,6 +156,9 @@ class FusedProgram:
         # skipped when the colour code leaves the state alone
         self.main_copy = not access.absorb_writes_main
         self.color_guard = access.color_is_pure
+        # stored colours from which every step's colour is its new value
+        # (None: unknown), and the new values' per-channel bounds
+        self.color_plan, self.color_values = _color_plan(model, access)
         # output tile rows split evenly over the block: column-owning
         # threads with compile-time region offsets (see _STEP_LOOP_SPLIT)
         groups = BLOCK // self.tile[0] if BLOCK % self.tile[0] == 0 else 0
@@ -161,8 +170,22 @@ class FusedProgram:
         if self.color_guard:
             absorb_body = (access.absorb_code + "if (_inner) {\n"
                            + access.color_code + "}\n" + access.store_code)
-        self.source = self._render(defines, emit_body, absorb_body, margin)
-        self._module = None
+        self.source = self._render(defines, emit_body, absorb_body, None,
+                                   margin)
+        # the form that computes the colour on a launch's last step only,
+        # without reading the stored colour: each step is held twice, with
+        # the colour and without it
+        self.source_fast = None
+        if self.color_plan is not None:
+            memory = type(model).color.memory
+            color_code = access.color_code.replace(memory, "") if memory \
+                else access.color_code
+            self.source_fast = self._render(
+                defines, emit_body,
+                access.absorb_code + "if (_inner) {\n" + color_code + "}\n"
+                + access.store_code,
+                access.absorb_code + access.store_code, margin)
+        self._module = self._module_fast = None
 
     def _instance_dtype(self):
         fields = [('cells', np.int64), ('cols', np.int64), ('w0', np.int32),
@@ -174,7 +197,11 @@ class FusedProgram:
             fields.append(('pad', np.int32))
         return np.dtype(fields, align=True)
 
-    def _render(self, defines, emit_body, absorb_body, margin):
+    def _render(self, defines, emit_body, absorb_body, absorb_plain,
+                margin):
+        """The kernel source; with ``absorb_plain`` (the absorb body
+        without the colour) the form that computes the colour only on a
+        launch's last step and never reads the stored colours."""
         tx, ty = self.tile
         params_struct = "".join(
             "%s p_%s;\n" % (_PARAM_FIELD[p.ctype][0], p.name)
@@ -192,6 +219,7 @@ class FusedProgram:
             "_tb[%d * _RA + _li] = (_word) _BVALUE;\n" % (slot - 1)
             for slot in range(1, self.k + 1))
         border = "1" if self.static else "0"
+        cfast = 0 if absorb_plain is None else 1
         if self.folded:
             zero_slots = tb_decl = emit_section = ""
         else:
@@ -201,26 +229,40 @@ class FusedProgram:
                 static_emit=static_emit, emit_body=emit_body)
         main_copy = ("                _mp[_cur ^ 1][_li] = _mp[_cur][_li];"
                      if self.main_copy else "")
+        def both(with_color, without_color):
+            if absorb_plain is None:
+                return with_color
+            return ("if (_last) {\n" + with_color + "} else {\n"
+                    + without_color + "}\n")
+
         if self.split:
-            cells = "".join(
-                _SPLIT_CELL.format(j=j, main_copy=main_copy,
-                                   absorb_body=absorb_body)
-                for j in range(self.rows_per_thread))
+            def cells(body):
+                return "".join(
+                    _SPLIT_CELL.format(j=j, main_copy=main_copy,
+                                       absorb_body=body)
+                    for j in range(self.rows_per_thread))
             run = self.ring_run
+            # ring cells never compute the colour
+            ring_body = absorb_plain or absorb_body
             step_body = _SPLIT_STEP.format(
-                inner_cells=cells,
-                vertical_run=_ring_run("_RW", run, main_copy, absorb_body),
-                horizontal_run=_ring_run("1", run, main_copy, absorb_body))
+                inner_cells=both(cells(absorb_body), cells(absorb_plain)),
+                vertical_run=_ring_run("_RW", run, main_copy, ring_body),
+                horizontal_run=_ring_run("1", run, main_copy, ring_body))
             step_loop = _STEP_LOOP_SPLIT.format(step_body=step_body)
         else:
+            def absorb_pass(body):
+                return _GENERIC_PASS.format(main_copy=main_copy,
+                                            absorb_body=body)
             step_loop = _STEP_LOOP_GENERIC.format(
-                emit_section=emit_section, main_copy=main_copy,
-                absorb_body=absorb_body)
+                emit_section=emit_section,
+                absorb_pass=both(absorb_pass(absorb_body),
+                                 absorb_pass(absorb_plain)))
         return _KERNEL_TEMPLATE.format(
             defines=defines, tx=tx, ty=ty, margin=margin,
             k=0 if self.folded else self.k, nt=BLOCK,
             rows=self.rows_per_thread, run=self.ring_run,
             he=self.halo_per_step, ctype=self.ctype, static=border,
+            cfast=cfast,
             bvalue="%dULL" % (self.border_value & 0xFFFFFFFFFFFFFFFF),
             params_struct=params_struct, param_locals=param_locals,
             zero_slots=zero_slots, tb_decl=tb_decl, step_loop=step_loop)
@@ -231,6 +273,38 @@ class FusedProgram:
         if self._module is None:
             self._module = hip.Module.from_source(self.source, "bsca_fused.hip")
         return self._module.get_function("fused_bsca")
+
+    @property
+    def function_fast(self):
+        """The kernel that computes the colour on a launch's last step."""
+        if self._module_fast is None:
+            self._module_fast = hip.Module.from_source(
+                self.source_fast, "bsca_fused_fast.hip")
+        return self._module_fast.get_function("fused_bsca")
+
+    def admits(self, bounds):
+        """Whether stored colours within ``bounds`` (per-channel ``(min,
+        max)``, None if unknown) allow :attr:`function_fast`."""
+        plan = self.color_plan
+        if plan is None:
+            return False
+        if plan == ():
+            return True
+        return bounds is not None and all(
+            low <= lo and hi <= high
+            for (lo, hi), (low, high) in zip(bounds, plan))
+
+    def bounds_after(self, bounds, fast):
+        """Colour bounds after a launch from colours within ``bounds``."""
+        if self.color_values is not None and (fast or self.admits(bounds)):
+            return self.color_values
+        return None
+
+    def kernel_for(self, bounds):
+        """``(fast, function)`` to launch on colours within ``bounds``."""
+        if self.admits(bounds):
+            return True, self.function_fast
+        return False, self.function
 
     def tiles_of(self, size):
         """``(tiles_x, tiles_y)`` covering a field of ``size``."""
@@ -253,6 +327,10 @@ _KERNEL_TEMPLATE = r"""
 #define _HE {he}
 #define _STATIC {static}
 #define _BVALUE {bvalue}
+// 1: only a launch's last step computes the colour, and the stored
+// colours are not read (launched only where the colour written provably
+// equals the new value at every step); 0: every step computes it
+#define _CFAST {cfast}
 typedef {ctype} _word;
 
 // The cells of the region at distance >= margin from its edge, one per
@@ -357,7 +435,8 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
             {zero_slots}
         }}
 #endif
-#if _NT % _TX == 0
+#if _CFAST
+#elif _NT % _TX == 0
         {{ // colours of the output tile: thread (column _cx, phase _ry) loads
            // rows _ry, _ry + _NT / _TX, ... inside the field, eight at a time
             const int _cx = (int) threadIdx.x % _TX, _gx = _ox + _cx;
@@ -424,7 +503,16 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
 _STEP_LOOP_GENERIC = r"""        int _cur = 0;
         for (int _s = 0; _s < _steps; ++_s) {{
             const int _da = (_s + 1) * _HE;
+            const bool _last = _s + 1 == _steps;
 {emit_section}
+{absorb_pass}
+            __syncthreads();
+            _cur ^= 1;
+        }}
+"""
+
+# The absorb pass of one step of the general form.
+_GENERIC_PASS = r"""
             {{ _RECT_FOR(_da) {{
 #if _STATIC
                 const int _gx = _ox - _M + _lx, _gy = _oy - _M + _ly;
@@ -443,9 +531,6 @@ _STEP_LOOP_GENERIC = r"""        int _cur = 0;
 {absorb_body}
                 }}
             }} }}
-            __syncthreads();
-            _cur ^= 1;
-        }}
 """
 
 # The step loop of a folded program whose tile rows split evenly over the
@@ -459,11 +544,13 @@ _STEP_LOOP_SPLIT = r"""
         int _cur = 0;
         for (int _s = 0; _s < _steps; _s += 2) {{
             {{ const int _cur = 0, _da = (_s + 1) * _HE;
+            const bool _last = _s + 1 == _steps;
 {step_body}
             }}
             __syncthreads();
             if (_s + 1 >= _steps) break;
             {{ const int _cur = 1, _da = (_s + 2) * _HE;
+            const bool _last = _s + 2 == _steps;
 {step_body}
             }}
             __syncthreads();
@@ -588,6 +675,95 @@ _EMIT_SECTION = r"""
 """
 
 
+def _color_plan(model, access):
+    """
+    When a launch may compute the colour on its last step only.
+
+    Returns ``(plan, values)``.  ``plan`` is None when every step must
+    (the colour code changes the state, or nothing is known of its
+    values); ``()`` when the colour written never reads the stored one; or
+    one ``(low, high)`` pair per channel: from stored colours within these
+    the colour of every step is its new value, so only the last step's
+    counts.  ``values`` bounds the new values per channel (None if not
+    known); they lie within the plan's intervals again.
+
+    For ``MovingAverage`` the pairs are ``[hi - FADE_IN, lo + FADE_OUT]``
+    where ``[lo, hi]`` bounds the channel's new value: from a stored colour
+    in that interval the clamp returns the new value.
+    """
+    nothing = (None, None)
+    if not access.color_is_pure:
+        return nothing
+    effect = getattr(type(model), "color", None)
+    if not isinstance(effect, ColorEffect):
+        return nothing
+    if not effect.memory:
+        return (), None
+    code = access.color_code
+    if code.count(effect.memory) != 1:
+        return nothing
+    fades = []
+    for name in ("FADE_IN", "FADE_OUT"):
+        const = model.constants.get(name)
+        value = None if const is None else const.value
+        if not isinstance(value, int) or isinstance(value, bool):
+            return nothing
+        fades.append(value)
+    fade_in, fade_out = fades
+    bounds = ranges.channel_bounds(
+        access.absorb_code + code[:code.index(effect.memory)],
+        *_range_context(model), channels=("new_r", "new_g", "new_b"))
+    if bounds is None:
+        return nothing
+    plan = []
+    for lo, hi in bounds:
+        if hi - lo > fade_in or hi - lo > fade_out:
+            return nothing
+        low, high = hi - fade_in, lo + fade_out
+        # the clamp's old - FADE_OUT and old + FADE_IN stay in int range
+        if low - fade_out < -(1 << 31) or high + fade_in >= 1 << 31:
+            return nothing
+        plan.append((low, high))
+    return tuple(plan), tuple(bounds)
+
+
+def _range_context(model):
+    """Names, arrays and typedefs the tile code of ``model`` refers to."""
+    analysis = ranges.Analysis()
+    names = {}
+    for name, const in model.constants.items():
+        if isinstance(const.value, bool) or not isinstance(
+                const.value, (int, float)):
+            continue
+        try:
+            names[name] = analysis.evaluate(str(const.value))
+        except (ranges.Unknown, ranges.GiveUp):
+            continue
+    params = {p.name: p for p in model._emit_params + model._absorb_params}
+    for name, param in params.items():
+        ctype = ranges.CTYPES.get((param.ctype, ))
+        if ctype is not None:
+            names[name] = ranges.Value.full(ctype)
+    int_type = ranges.CTYPES[("int", )]
+    for name in ("_li", "_cur", "_RW", "_RH", "_RA", "_TX", "_TY", "_M"):
+        names[name] = ranges.Value.full(int_type)
+    names["i"] = ranges.Value(ranges.CTYPES[("unsigned", )], 0, 0)
+    word = ranges.type_named(model.main.ctype)
+    types = {"_word": word} if word is not None else {}
+    pointers = {"_mp": (word, 2), "_tb": (word, 1)} if word is not None \
+        else {}
+    return names, pointers, types
+
+
+def _union(bounds_list):
+    """Per-channel bounds covering every entry (None if one is unknown)."""
+    if not bounds_list or any(b is None for b in bounds_list):
+        return None
+    return tuple((min(b[c][0] for b in bounds_list),
+                  max(b[c][1] for b in bounds_list))
+                 for c in range(len(bounds_list[0])))
+
+
 def _persistent_grid(n_tiles, lds_bytes):
     """Workgroups of a persistent launch: all CUs, as many as LDS allows."""
     per_cu = max(1, min(8, (160 * 1024) // max(1, lds_bytes)))
@@ -705,13 +881,14 @@ class FusedEngine:
         colors = arrays.colors.device_ptr()
         src, dst = ((self._other.ptr, plane0) if self._in_other
                     else (plane0, self._other.ptr))
-        # launches prepared once per (direction, step count, arrays)
-        key = (src, dst, colors, steps)
+        fast, function = self.program.kernel_for(arrays.colors.bounds)
+        # launches prepared once per (direction, step count, arrays, form)
+        key = (src, dst, colors, steps, fast)
         launch = self._launches.get(key)
         if launch is None:
             grid = _persistent_grid(self._inst.n_tiles,
                                     self.program.lds_bytes)
-            launch = self.program.function.prepare(
+            launch = function.prepare(
                 np.uint64(src), np.uint64(dst), np.uint64(colors),
                 np.uint64(self._inst.table_buf.ptr),
                 np.uint64(self._inst.tiles_buf.ptr),
@@ -721,6 +898,8 @@ class FusedEngine:
                 self._launches.clear()
             self._launches[key] = launch
         launch()
+        arrays.colors.bounds = self.program.bounds_after(
+            arrays.colors.bounds, fast)
         self._in_other = not self._in_other
 
     def flush(self):
@@ -774,6 +953,7 @@ class Batch:
         total = self.inst.total_cells * self.inst.itemsize
         self.state = [hip.Buffer(total), hip.Buffer(total)]
         self.colors = hip.Buffer(12 * self.inst.total_cells)
+        self.color_bounds = None
         self.cur = 0
         self._graphs = {}
         self.gather()
@@ -794,6 +974,8 @@ class Batch:
                 self.colors.ptr + coff * 12, arrays.colors.device_ptr(),
                 model.cells_num * 12, self.stream.handle))
         self.stream.synchronize()
+        self.color_bounds = _union(
+            [model.gpu.arrays.colors.bounds for model in self.models])
 
     def scatter(self):
         """Copy the batch state back into every model's arrays."""
@@ -809,11 +991,14 @@ class Batch:
             check(lib.xen_memcpy_d2d(
                 arrays.colors.device_ptr(), self.colors.ptr + coff * 12,
                 model.cells_num * 12, self.stream.handle))
+            arrays.colors.bounds = self.color_bounds
         self.stream.synchronize()
 
-    def _launch(self, steps):
+    def _launch(self, steps, fast):
         grid = _persistent_grid(self.inst.n_tiles, self.program.lds_bytes)
-        self.program.function(
+        function = (self.program.function_fast if fast
+                    else self.program.function)
+        function(
             np.uint64(self.state[self.cur].ptr),
             np.uint64(self.state[1 - self.cur].ptr),
             np.uint64(self.colors.ptr), np.uint64(self.inst.table_buf.ptr),
@@ -831,19 +1016,23 @@ class Batch:
     def run(self, steps, graph=True):
         """Advance every instance ``steps`` steps (asynchronous)."""
         self.inst.refresh()
-        self.program.function  # compile before capture
+        fast, function = self.program.kernel_for(self.color_bounds)
         plan = self.launch_plan(steps)
+        if not plan:
+            return
+        bounds = self.program.bounds_after(self.color_bounds, fast)
         if not graph:
             for count in plan:
-                self._launch(count)
+                self._launch(count, fast)
+            self.color_bounds = bounds
             return
-        key = (tuple(plan), self.cur)
+        key = (tuple(plan), self.cur, fast)
         if key not in self._graphs:
             start = self.cur
             hip.Graph.begin(self.stream)
             try:
                 for count in plan:
-                    self._launch(count)
+                    self._launch(count, fast)
             finally:
                 graph_obj = hip.Graph.end(self.stream)
             self._graphs[key] = (graph_obj, self.cur)
@@ -851,6 +1040,7 @@ class Batch:
         graph_obj, end = self._graphs[key]
         graph_obj.launch(self.stream)
         self.cur = end
+        self.color_bounds = bounds
 
     def synchronize(self):
         """Wait for the batch's stream."""
