@@ -246,3 +246,16 @@ def test_colour_bounds_follow_host_writes():
     data[3 * 17 + 2] = -3
     colors.set(data)
     assert colors.bounds == ((0, 0), (0, 700), (-3, 0))
+
+
+def test_stores_only_into_arrays():
+    word = ranges.type_named("unsigned char")
+    analysis = ranges.Analysis({}, {"_mp": (word, 2)}, {})
+    analysis.run("int a = 1;\n(_mp[_cur ^ 1])[_li] = a;\ncol[i] = a;\n")
+    assert _iv(analysis.env["a"]) == (1, 1)
+    # a store through a local's address would change the local
+    for code in ("int a = 1;\n(* reinterpret_cast<float*>(&(a))) = 2.0f;\n",
+                 "int a = 1;\nother[0] = a;\n",
+                 "int a = 1;\n*(&a) = 5;\n"):
+        with pytest.raises(ranges.GiveUp):
+            ranges.Analysis({}, {"_mp": (word, 2)}, {}).run(code)

@@ -339,7 +339,8 @@ _TOKEN = re.compile(r"""\s*(?:
     (?P<float>(?:\d+\.\d*|\.\d+)(?:[eE][-+]?\d+)?[fF]?|\d+[eE][-+]?\d+[fF]?)
   | (?P<int>0[xX][0-9a-fA-F]+[uUlL]*|\d+[uUlL]*)
   | (?P<name>[A-Za-z_]\w*)
-  | (?P<op><<=|>>=|<<|>>|<=|>=|==|!=|&&|\|\||[-+*/%&|^]=|[-+*/%&|^~!<>?:()\[\],.=;{}])
+  | (?P<op><<=|>>=|<<|>>|<=|>=|==|!=|&&|\|\||[-+*/%&|^]=
+         |[-+*/%&|^~!<>?:()\[\],.=;{}])
 )""", re.VERBOSE)
 
 _PRECEDENCE = {
@@ -554,12 +555,15 @@ class Analysis:
     :param pointers: ``{identifier: (CType, depth)}`` of the arrays the
         code reads (their elements span their whole type)
     :param types: ``{type name: CType}`` of typedefs (``_word``)
+    :param stores: names of further arrays the code stores into (``col``)
     """
 
-    def __init__(self, names=None, pointers=None, types=None):
+    def __init__(self, names=None, pointers=None, types=None,
+                 stores=("col", )):
         self.env = dict(names or {})
         self.pointers = dict(pointers or {})
         self.types = dict(types or {})
+        self.arrays = set(self.pointers) | set(stores)
         self.declared = {}
 
     def type_of(self, name):
@@ -647,10 +651,15 @@ class Analysis:
             except Unknown:
                 self.env[name] = Value.full(ctype)
             return
-        # a store through an array or pointer changes no local
-        if tokens and (tokens[0][1] == "(" or len(tokens) > 1
-                       and tokens[0][0] == "name" and tokens[1][1] == "["):
-            if "=" in (text for _, text in tokens):
+        # a store into one of the arrays (never a local: its target starts
+        # with an array's name and takes no address) changes no local
+        texts = [text for _, text in tokens]
+        if "=" in texts:
+            target = tokens[:texts.index("=")]
+            names = [text for kind, text in target if kind == "name"]
+            if (names and names[0] in self.arrays
+                    and "&" not in texts[:len(target)]
+                    and target[-1][1] == "]"):
                 return
         raise GiveUp("statement %r" % stmt)
 
