@@ -165,6 +165,25 @@ def test_batch_with_colours_outside_the_plan():
         assert model.gpu.arrays.colors.bounds is None
 
 
+def test_set_after_deferred_steps():
+    """Steps the fused engine has deferred run before a ``set()`` of the
+    cells or colours, as the plain engine's steps have."""
+    plain = GameOfLife(GOLExperiment2)
+    fast = GameOfLife(fused(GOLExperiment2))
+    rng = np.random.default_rng(11)
+    cells = plain.gpu.arrays.cells.get()
+    cells[:plain.cells_num] = rng.integers(0, 2, plain.cells_num)
+    colors = rng.integers(0, 256, plain.cells_num * 3).astype(np.int32)
+    for model in (plain, fast):
+        run(model, 3)
+        model.gpu.arrays.cells.set(cells)
+        model.gpu.arrays.colors.set(colors)
+        run(model, 5)
+    assert crc(fast) == crc(plain)
+    assert np.array_equal(fast.gpu.arrays.colors.get(),
+                          plain.gpu.arrays.colors.get())
+
+
 def test_memoryless_colour_only_last_step():
     """A plain colour effect never reads the stored colour: the fused
     kernel computes it on a launch's last step and does not load it."""

@@ -259,3 +259,75 @@ def test_stores_only_into_arrays():
                  "int a = 1;\n*(&a) = 5;\n"):
         with pytest.raises(ranges.GiveUp):
             ranges.Analysis({}, {"_mp": (word, 2)}, {}).run(code)
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_selects_equal_products_against_gcc(tmp_path):
+    """Expressions written out again with 0/1 products as selects give the
+    original's value on every input (gcc evaluates both)."""
+    rng = random.Random(777)
+    cases = []
+    while len(cases) < 300:
+        types = [rng.choice(list(_LIMITS)) for _ in range(3)]
+        spans = [rng.choice([(0, 1), (0, 1), (0, 2), _interval(rng, t),
+                             _interval(rng, t)]) for t in types]
+        text = _expr(rng, 3)
+        if "*" not in text:
+            continue
+        names = {n: Value(_T[t], lo, hi)
+                 for n, t, (lo, hi) in zip("xyz", types, spans)}
+        try:
+            _, out = ranges.Analysis(names).evaluate(text, selects=True)
+        except ranges.Unknown:
+            continue
+        cases.append((types, spans, text, out))
+    assert sum(" ? (" in out for *_, out in cases) > 100
+    body = ["#include <stdio.h>", "int main(void) {"]
+    for idx, (types, spans, text, out) in enumerate(cases):
+        (tx, ty, tz), ((xl, xh), (yl, yh), (zl, zh)) = types, spans
+        body.append(
+            "{ long bad = 0;\n"
+            "for (__int128 a = %s; a <= %s; ++a)\n"
+            "for (__int128 b = %s; b <= %s; ++b)\n"
+            "for (__int128 c = %s; c <= %s; ++c) {\n"
+            "  %s x = (%s) a; %s y = (%s) b; %s z = (%s) c;\n"
+            "  long double u = (long double) (%s);\n"
+            "  long double v = (long double) (%s);\n"
+            "  if (u != v && (u == u || v == v)) ++bad;\n"
+            "  (void) x; (void) y; (void) z; }\n"
+            "printf(\"%d %%ld\\n\", bad); }"
+            % (_c_int128(xl), _c_int128(xh), _c_int128(yl), _c_int128(yh),
+               _c_int128(zl), _c_int128(zh), tx, tx, ty, ty, tz, tz, text,
+               out, idx))
+    body.append("return 0; }")
+    src = tmp_path / "selects.c"
+    src.write_text("\n".join(body))
+    exe = tmp_path / "selects"
+    subprocess.run(["gcc", "-O0", "-fwrapv", "-w", "-o", str(exe), str(src)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True,
+                         text=True).stdout.split()
+    results = dict(zip(out[::2], out[1::2]))
+    assert len(results) == len(cases)
+    bad = {idx: cases[int(idx)][2:] for idx, count in results.items()
+           if count != "0"}
+    assert not bad
+
+
+def test_rewrite_statements():
+    analysis = ranges.Analysis({"rule": Value.full(_T["int"])})
+    code = ("unsigned char m = 300 & 1;\nunsigned int g = 70000;\n"
+            "unsigned int b = 0;\nb += ((m * (g > b)) * (g - b));\n"
+            "float f = 0.0f;\nf = (m * 2.5f);\n")
+    out, complete = analysis.rewrite(code)
+    assert complete
+    assert out == (
+        "\nunsigned char m = (300 & 1);\nunsigned int g = 70000;\n"
+        "unsigned int b = 0;\n"
+        "b += ((((m ? (int) ((g > b)) : (int) 0)) ? (unsigned int) ((g - b))"
+        " : (unsigned int) 0));\n"
+        # float products stay products
+        "float f = 0.0f;\nf = ((m * 2.5f));\n")
+    out, complete = ranges.Analysis({}).rewrite(
+        "int a = 1;\nif (a) { a = 2; }\nint b = (a * 3);\n")
+    assert not complete and out.endswith("int b = (a * 3);\n")

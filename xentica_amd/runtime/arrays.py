@@ -7,7 +7,8 @@ reads ``model.gpu.arrays.cells.get()``).
 An array stays on the host until a kernel first needs its device address,
 so models can be built, seeded and inspected on machines without a GPU.
 An owner may attach a ``before_read`` hook that runs ahead of every
-``.get()``; the engines use it to finish deferred steps first.
+``.get()`` and ``.set()``; the engines use it to finish deferred steps
+first, so that a read sees them and a write is not overwritten by them.
 
 An array of interleaved channels (the colours: red, green, blue per cell)
 can keep per-channel value bounds: computed from the data at every
@@ -78,6 +79,8 @@ class DeviceArray:
     def set(self, host):
         """Replace the contents with ``host`` (same size and dtype)."""
         host = np.ascontiguousarray(host, dtype=self.dtype).reshape(self.shape)
+        if self.before_read is not None:
+            self.before_read()
         if self._buffer is None:
             self._host = host.copy()
         else:

@@ -159,6 +159,8 @@ class FusedProgram:
         # stored colours from which every step's colour is its new value
         # (None: unknown), and the new values' per-channel bounds
         self.color_plan, self.color_values = _color_plan(model, access)
+        # products of an integer 0 or 1 written as selects
+        absorb_code = _with_selects(model, access.absorb_code)
         # output tile rows split evenly over the block: column-owning
         # threads with compile-time region offsets (see _STEP_LOOP_SPLIT)
         groups = BLOCK // self.tile[0] if BLOCK % self.tile[0] == 0 else 0
@@ -168,7 +170,7 @@ class FusedProgram:
         # cells per ring run unrolled in the kernel (longer runs loop)
         self.ring_run = max(1, min(self.halo_per_step * self.depth - 1, 8))
         if self.color_guard:
-            absorb_body = (access.absorb_code + "if (_inner) {\n"
+            absorb_body = (absorb_code + "if (_inner) {\n"
                            + access.color_code + "}\n" + access.store_code)
         self.source = self._render(defines, emit_body, absorb_body, None,
                                    margin)
@@ -182,9 +184,9 @@ class FusedProgram:
                 else access.color_code
             self.source_fast = self._render(
                 defines, emit_body,
-                access.absorb_code + "if (_inner) {\n" + color_code + "}\n"
+                absorb_code + "if (_inner) {\n" + color_code + "}\n"
                 + access.store_code,
-                access.absorb_code + access.store_code, margin)
+                absorb_code + access.store_code, margin)
         self._module = self._module_fast = None
 
     def _instance_dtype(self):
@@ -725,6 +727,17 @@ def _color_plan(model, access):
             return nothing
         plan.append((low, high))
     return tuple(plan), tuple(bounds)
+
+
+def _with_selects(model, code):
+    """
+    ``code`` with every product of integers one of which is provably 0 or
+    1 written as a select: one ``v_cndmask`` where a 32-bit multiply (the
+    DSL's idiom ``flag * value``) is a quarter-rate instruction.  The
+    values are those of the C expressions (``runtime/ranges.py``).
+    """
+    text, _ = ranges.Analysis(*_range_context(model)).rewrite(code)
+    return text
 
 
 def _range_context(model):

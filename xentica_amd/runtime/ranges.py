@@ -78,11 +78,13 @@ CTYPES = {
     ("long", "long"): _LONG,
     ("unsigned", "long"): _ULONG,
     ("unsigned", "long", "long"): _ULONG,
+    ("__int128", ): CType("__int128", 128, True),
+    ("unsigned", "__int128"): CType("unsigned __int128", 128, False),
     ("float", ): _FLOAT,
     ("double", ): _DOUBLE,
 }
 _TYPE_WORDS = {"bool", "char", "signed", "unsigned", "short", "int", "long",
-               "float", "double"}
+               "float", "double", "__int128"}
 
 # float results are widened outward by this relative amount (a float32
 # rounding is 2^-24 relative) and by the smallest normal float
@@ -391,13 +393,21 @@ def _int_literal(text):
 
 
 class _Parser:
-    """Evaluates one expression's tokens over an environment of ranges."""
+    """
+    Evaluates one expression's tokens over an environment of ranges, and
+    writes the expression out again: fully parenthesised, and with
+    ``selects`` a product of integers one of which is 0 or 1 as a select
+    (``a * b`` with ``a`` in [0, 1] is ``a ? (T) b : (T) 0``, ``T`` the
+    product's type), which the GPU does in one instruction where a 32-bit
+    multiply takes four.
+    """
 
-    def __init__(self, tokens, lookup, type_of):
+    def __init__(self, tokens, lookup, type_of, selects=False):
         self.tokens = tokens
         self.pos = 0
         self.lookup = lookup
         self.type_of = type_of
+        self.selects = selects
 
     def peek(self, offset=0):
         idx = self.pos + offset
@@ -411,38 +421,46 @@ class _Parser:
         return tok
 
     def expression(self):
-        cond = self.binary(1)
+        cond, ctext = self.binary(1)
         if self.peek()[1] != "?":
-            return cond
+            return cond, ctext
         self.take("?")
-        first = self.expression()
+        first, ftext = self.expression()
         self.take(":")
-        second = self.expression()
-        if _is_pointer(first) or _is_pointer(second):
+        second, stext = self.expression()
+        if _is_pointer(cond) or _is_pointer(first) or _is_pointer(second):
             raise Unknown("pointer in ?:")
         ctype = _common(first.ctype, second.ctype)
         first, second = convert(first, ctype), convert(second, ctype)
-        return Value(ctype, min(first.lo, second.lo), max(first.hi, second.hi))
+        return (Value(ctype, min(first.lo, second.lo),
+                      max(first.hi, second.hi)),
+                "(%s ? %s : %s)" % (ctext, ftext, stext))
 
     def binary(self, level):
-        left = self.unary()
+        left, ltext = self.unary()
         while True:
             op = self.peek()[1]
             prec = _PRECEDENCE.get(op) if self.peek()[0] == "op" else None
             if prec is None or prec < level:
-                return left
+                return left, ltext
             self.take()
-            right = self.binary(prec + 1)
+            right, rtext = self.binary(prec + 1)
+            text = "(%s %s %s)" % (ltext, op, rtext)
             if _is_pointer(left):
                 if op in ("+", "-") and not _is_pointer(right):
-                    continue   # pointer arithmetic keeps the pointer
+                    ltext = text   # pointer arithmetic keeps the pointer
+                    continue
                 raise Unknown("pointer operand")
             if _is_pointer(right):
                 raise Unknown("pointer operand")
-            left = binary(op, left, right)
+            value = binary(op, left, right)
+            if op == "*" and self.selects:
+                text = _select_product(left, ltext, right, rtext,
+                                       value.ctype) or text
+            left, ltext = value, text
 
     def cast_type(self):
-        """The type of a ``(type)`` cast at the cursor, or None."""
+        """The type (and its words) of a ``(type)`` cast at the cursor."""
         idx = 1
         words = []
         while True:
@@ -455,77 +473,80 @@ class _Parser:
                 continue
             break
         if not words or self.peek(idx)[1] != ")":
-            return None
+            return None, None
         if len(words) == 1 and self.type_of(words[0]) is not None:
             ctype = self.type_of(words[0])
         else:
             ctype = _name_type(words)
         if ctype is None:
-            return None
+            return None, None
         self.pos += idx + 1
-        return ctype
+        return ctype, " ".join(words)
 
     def unary(self):
         kind, text = self.peek()
         if kind == "op" and text in ("-", "+", "!", "~"):
             self.take()
-            operand = self.unary()
+            operand, otext = self.unary()
             if _is_pointer(operand):
                 raise Unknown("pointer operand")
-            return unary(text, operand)
+            return unary(text, operand), "(%s%s)" % (text, otext)
         if kind == "op" and text in ("*", "&"):
             raise Unknown("dereference or address")
         if kind == "op" and text == "(":
-            ctype = self.cast_type()
+            ctype, words = self.cast_type()
             if ctype is not None:
-                operand = self.unary()
+                operand, otext = self.unary()
                 if _is_pointer(operand):
                     raise Unknown("pointer cast")
-                return convert(operand, ctype)
+                return convert(operand, ctype), "((%s) %s)" % (words, otext)
         return self.postfix()
 
     def postfix(self):
-        value = self.primary()
+        value, text = self.primary()
         while True:
-            text = self.peek()[1]
-            if text == "[":
-                self.skip_brackets("[", "]")
+            nxt = self.peek()[1]
+            if nxt == "[":
+                text += self.skip_brackets("[", "]")
                 if not _is_pointer(value):
                     raise Unknown("subscript of a value")
                 if value.depth == 1:
                     value = Value.full(value.ctype)
                 else:
                     value = _Pointer(value.ctype, value.depth - 1)
-            elif text in (".", "("):
+            elif nxt in (".", "("):
                 raise Unknown("member or call")
             else:
-                return value
+                return value, text
 
     def skip_brackets(self, opening, closing):
+        """The tokens up to the matching ``closing``, as text."""
         depth = 0
+        texts = []
         while True:
             kind, text = self.take()
             if kind is None:
                 raise Unknown("unbalanced")
+            texts.append(text)
             if text == opening:
                 depth += 1
             elif text == closing:
                 depth -= 1
                 if depth == 0:
-                    return
+                    return " ".join(texts)
 
     def primary(self):
         kind, text = self.take()
         if kind == "int":
-            return _int_literal(text)
+            return _int_literal(text), text
         if kind == "float":
             ctype = _FLOAT if text[-1] in "fF" else _DOUBLE
             value = float(text.rstrip("fF"))
-            return _widen(ctype, value, value)
+            return _widen(ctype, value, value), text
         if kind == "op" and text == "(":
-            value = self.expression()
+            value, inner = self.expression()
             self.take(")")
-            return value
+            return value, "(%s)" % inner
         if kind == "name":
             if self.peek()[1] == "(" and text in _FUNCTIONS:
                 count, func = _FUNCTIONS[text]
@@ -535,11 +556,24 @@ class _Parser:
                     self.take(",")
                     args.append(self.expression())
                 self.take(")")
-                if len(args) != count or any(_is_pointer(a) for a in args):
+                if len(args) != count or any(
+                        _is_pointer(a) for a, _ in args):
                     raise Unknown(text)
-                return func(*args)
-            return self.lookup(text)
+                return (func(*(a for a, _ in args)),
+                        "%s(%s)" % (text, ", ".join(t for _, t in args)))
+            return self.lookup(text), text
         raise Unknown("token %r" % text)
+
+
+def _select_product(left, ltext, right, rtext, ctype):
+    """``left * right`` as a select when one integer factor is 0 or 1."""
+    if ctype.floating or left.ctype.floating or right.ctype.floating:
+        return None
+    for cond, ctext, other in ((left, ltext, rtext), (right, rtext, ltext)):
+        if cond.lo >= 0 and cond.hi <= 1:
+            return "(%s ? (%s) %s : (%s) 0)" % (ctext, ctype.name, other,
+                                               ctype.name)
+    return None
 
 
 def _is_pointer(value):
@@ -580,19 +614,21 @@ class Analysis:
             raise Unknown(name)
         return value
 
-    def evaluate(self, text):
-        """The range of a C expression (raises :class:`Unknown`)."""
-        parser = _Parser(tokenize(text), self.lookup, self.type_of)
-        value = parser.expression()
+    def evaluate(self, text, selects=False):
+        """The range of a C expression (raises :class:`Unknown`); with
+        ``selects``, ``(range, text written out again)``."""
+        parser = _Parser(tokenize(text), self.lookup, self.type_of, selects)
+        value, out = parser.expression()
         if parser.pos != len(parser.tokens) or _is_pointer(value):
             raise Unknown("trailing tokens")
-        return value
+        return (value, out) if selects else value
 
     def _eval_or_full(self, text, ctype):
         try:
-            return convert(self.evaluate(text), ctype)
+            value, out = self.evaluate(text, selects=True)
+            return convert(value, ctype), out
         except Unknown:
-            return Value.full(ctype)
+            return Value.full(ctype), text.strip()
 
     def run(self, code):
         """Follow every statement of ``code`` (raises on unknown shapes)."""
@@ -601,8 +637,31 @@ class Analysis:
             if stmt:
                 self.statement(stmt)
 
+    def rewrite(self, code):
+        """
+        ``(text, complete)``: ``code`` with every product of integers one
+        of which the ranges show to be 0 or 1 written as a select, and
+        whether every statement was followed (the statements from the
+        first of unknown shape on are kept as they are).  Follows the code
+        as :meth:`run` does.
+        """
+        out = []
+        pieces = code.split(";")
+        for idx, raw in enumerate(pieces):
+            stmt = raw.strip()
+            if not stmt:
+                out.append(raw)
+                continue
+            try:
+                out.append("\n" + self.statement(stmt))
+            except GiveUp:
+                out.append(";".join(pieces[idx:]))
+                return ";".join(out), False
+        return ";".join(out), True
+
     def statement(self, stmt):
-        """Follow one statement (without its ``;``)."""
+        """Follow one statement (without its ``;``); returns it written
+        out again with products of a 0/1 factor as selects."""
         tokens = tokenize(stmt)
         if any(text in ("{", "}") or text in ("if", "for", "while", "do",
                                                 "return", "goto", "switch")
@@ -628,12 +687,12 @@ class Analysis:
             self.declared[name] = ctype
             if not rest:
                 self.env[name] = Value.full(ctype)
-                return
+                return stmt
             if rest[0][1] != "=":
                 raise GiveUp("declaration %r" % stmt)
-            self.env[name] = self._eval_or_full(
-                stmt[stmt.index("=") + 1:], ctype)
-            return
+            cut = stmt.index("=")
+            self.env[name], out = self._eval_or_full(stmt[cut + 1:], ctype)
+            return "%s= %s" % (stmt[:cut], out)
         if tokens and tokens[0][0] == "name" and len(tokens) > 1 and (
                 tokens[1][1] == "=" or tokens[1][1].endswith("=")
                 and tokens[1][1] not in ("==", "!=", "<=", ">=")):
@@ -641,16 +700,19 @@ class Analysis:
             if name not in self.declared:
                 raise GiveUp("assignment to undeclared %r" % name)
             ctype = self.declared[name]
-            rhs = stmt[stmt.index(op) + len(op):]
+            cut = stmt.index(op)
+            rhs = stmt[cut + len(op):]
             if op == "=":
-                self.env[name] = self._eval_or_full(rhs, ctype)
-                return
+                self.env[name], out = self._eval_or_full(rhs, ctype)
+                return "%s= %s" % (stmt[:cut], out)
             try:
-                value = binary(op[:-1], self.env[name], self.evaluate(rhs))
-                self.env[name] = convert(value, ctype)
+                value, out = self.evaluate(rhs, selects=True)
+                self.env[name] = convert(
+                    binary(op[:-1], self.env[name], value), ctype)
             except Unknown:
                 self.env[name] = Value.full(ctype)
-            return
+                out = rhs.strip()
+            return "%s%s %s" % (stmt[:cut], op, out)
         # a store into one of the arrays (never a local: its target starts
         # with an array's name and takes no address) changes no local
         texts = [text for _, text in tokens]
@@ -660,7 +722,7 @@ class Analysis:
             if (names and names[0] in self.arrays
                     and "&" not in texts[:len(target)]
                     and target[-1][1] == "]"):
-                return
+                return stmt
         raise GiveUp("statement %r" % stmt)
 
 
