@@ -59,6 +59,13 @@ int xen_launch(xen_function function,
                unsigned block_x, unsigned block_y, unsigned block_z,
                unsigned shared_bytes, xen_stream stream, void **args);
 
+/* Workgroups of `block` threads one CU holds at once for `function` (its
+ * registers and LDS counted), and the device's CU count: the size of a
+ * persistent grid.  No counterpart in the reference, whose kernels are
+ * grid-stride launches sized by PyCUDA's GPUArray (base.py:713-721). */
+int xen_occupancy(xen_function function, unsigned block, unsigned shared_bytes,
+                  int *blocks_per_cu, int *cu_count);
+
 /* ---- memory (replaces gpuarray.to_gpu / zeros / .get(), base.py:544-569) */
 int xen_malloc(void **ptr, size_t bytes);
 int xen_free(void *ptr);

@@ -130,6 +130,25 @@ int xen_launch(xen_function function, unsigned grid_x, unsigned grid_y, unsigned
                  "hipModuleLaunchKernel");
 }
 
+int xen_occupancy(xen_function function, unsigned block, unsigned shared_bytes,
+                  int *blocks_per_cu, int *cu_count) {
+    int blocks = 0;
+    if (int rc = check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
+                           &blocks, reinterpret_cast<hipFunction_t>(function),
+                           static_cast<int>(block), shared_bytes),
+                       "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor"))
+        return rc;
+    int device = 0, cus = 0;
+    if (int rc = check(hipGetDevice(&device), "hipGetDevice")) return rc;
+    if (int rc = check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                             device),
+                       "hipDeviceGetAttribute"))
+        return rc;
+    *blocks_per_cu = blocks;
+    *cu_count = cus;
+    return 0;
+}
+
 int xen_malloc(void **ptr, size_t bytes) {
     *ptr = nullptr;
     if (bytes == 0) bytes = 1;

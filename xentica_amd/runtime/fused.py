@@ -178,15 +178,29 @@ class FusedProgram:
         # without reading the stored colour: each step is held twice, with
         # the colour and without it
         self.source_fast = None
+        self.lds_bytes_fast = self.lds_bytes
+        self.direct_colors = False
         if self.color_plan is not None:
             memory = type(model).color.memory
             color_code = access.color_code.replace(memory, "") if memory \
                 else access.color_code
+            # without the colour staging the LDS holds only the planes; the
+            # last step then stores each colour itself (a wave's 64 cells
+            # of a row are contiguous), which pays where it raises the
+            # workgroups a CU holds by half or more
+            planes_only = (2 + planes) * area * itemsize
+            if (_per_cu(planes_only) >= 1.5 * _per_cu(self.lds_bytes)
+                    and color_code.count(_COLOR_STORE) == 1):
+                self.direct_colors = True
+                self.lds_bytes_fast = planes_only
+                color_code = color_code.replace(_COLOR_STORE,
+                                                _COLOR_STORE_DIRECT)
             self.source_fast = self._render(
                 defines, emit_body,
                 absorb_code + "if (_inner) {\n" + color_code + "}\n"
                 + access.store_code,
-                absorb_code + access.store_code, margin)
+                absorb_code + access.store_code, margin,
+                cdirect=self.direct_colors)
         self._module = self._module_fast = None
 
     def _instance_dtype(self):
@@ -200,7 +214,7 @@ class FusedProgram:
         return np.dtype(fields, align=True)
 
     def _render(self, defines, emit_body, absorb_body, absorb_plain,
-                margin):
+                margin, cdirect=False):
         """The kernel source; with ``absorb_plain`` (the absorb body
         without the colour) the form that computes the colour only on a
         launch's last step and never reads the stored colours."""
@@ -264,7 +278,7 @@ class FusedProgram:
             k=0 if self.folded else self.k, nt=BLOCK,
             rows=self.rows_per_thread, run=self.ring_run,
             he=self.halo_per_step, ctype=self.ctype, static=border,
-            cfast=cfast,
+            cfast=cfast, cdirect=int(cdirect),
             bvalue="%dULL" % (self.border_value & 0xFFFFFFFFFFFFFFFF),
             params_struct=params_struct, param_locals=param_locals,
             zero_slots=zero_slots, tb_decl=tb_decl, step_loop=step_loop)
@@ -333,6 +347,8 @@ _KERNEL_TEMPLATE = r"""
 // colours are not read (launched only where the colour written provably
 // equals the new value at every step); 0: every step computes it
 #define _CFAST {cfast}
+// 1: the last step stores the colours straight to HBM (no LDS staging)
+#define _CDIRECT {cdirect}
 typedef {ctype} _word;
 
 // The cells of the region at distance >= margin from its edge, one per
@@ -377,7 +393,9 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
                 const int *__restrict__ _tile_inst, int _n_tiles, int _steps) {{
     __shared__ _word _mp[2][_RA];
     {tb_decl}
+#if !_CDIRECT
     __shared__ int3 _col[_TX * _TY + 256];
+#endif
     for (int _tile = blockIdx.x; _tile < _n_tiles; _tile += gridDim.x) {{
         const _Inst _in = _insts[_tile_inst[_tile]];
         const int _t = _tile - _in.tile0;
@@ -480,7 +498,9 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
                      _r += _NT / _TX) {{
                     const unsigned long long _g = _index(_oy + _r, _gx);
                     _dst[_g] = _mp[_cur][(_r + _M) * _RW + _cx + _M];
+#if !_CDIRECT
                     _cg[_g] = _col[_r * _TX + _cx];
+#endif
                 }}
         }}
 #else
@@ -490,7 +510,9 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
             if (_gx < _in.w0 && _gy < _in.w1) {{
                 const unsigned long long _g = _index(_gy, _gx);
                 _dst[_g] = _mp[_cur][(_qy + _M) * _RW + _qx + _M];
+#if !_CDIRECT
                 _cg[_g] = _col[_q];
+#endif
             }}
         }}
 #endif
@@ -528,7 +550,9 @@ _GENERIC_PASS = r"""
                 const int _oi = _inner ? (_ly - _M) * _TX + (_lx - _M)
                                        : _TX * _TY + (int) threadIdx.x;
                 {{
+#if !_CDIRECT
                     int3 *col = &_col[_oi];
+#endif
                     const unsigned i = 0;
 {absorb_body}
                 }}
@@ -613,7 +637,9 @@ _SPLIT_CELL = r"""
                     const bool _inner = true;
 {main_copy}
                     {{
-                        int3 *col = &_col[_oi];
+    #if !_CDIRECT
+                    int3 *col = &_col[_oi];
+#endif
                         const unsigned i = 0;
 {absorb_body}
                     }}
@@ -633,7 +659,9 @@ _RING_CELL = r"""
                                 const int _oi = _TX * _TY + (int) threadIdx.x;
 {main_copy}
                                 {{
+#if !_CDIRECT
                                     int3 *col = &_col[_oi];
+#endif
                                     const unsigned i = 0;
 {absorb_body}
                                 }}
@@ -777,10 +805,27 @@ def _union(bounds_list):
                  for c in range(len(bounds_list[0])))
 
 
-def _persistent_grid(n_tiles, lds_bytes):
-    """Workgroups of a persistent launch: all CUs, as many as LDS allows."""
-    per_cu = max(1, min(8, (160 * 1024) // max(1, lds_bytes)))
-    return max(1, min(n_tiles, 256 * per_cu))
+def _per_cu(lds_bytes):
+    """Workgroups of 256 threads a CU holds with ``lds_bytes`` of LDS each
+    (at most 8: 32 waves)."""
+    return max(1, min(8, (160 * 1024) // max(1, lds_bytes)))
+
+
+def _persistent_grid(n_tiles, function):
+    """Workgroups of a persistent launch: as many as the CUs hold at once
+    (a workgroup that waits for a free CU would finish a whole share of
+    tiles after the others)."""
+    per_cu, cus = function.occupancy(BLOCK)
+    return max(1, min(n_tiles, max(1, per_cu) * max(1, cus)))
+
+
+# the colour effect's store, and the last-step form's store straight to
+# HBM (output-tile cell _oi of the workgroup's tile, inside the field only)
+_COLOR_STORE = "col[i] = make_int3(new_r, new_g, new_b);"
+_COLOR_STORE_DIRECT = (
+    "{ const int _gxo = _ox + _oi % _TX, _gyo = _oy + _oi / _TX;\n"
+    "if (_gxo < _in.w0 && _gyo < _in.w1)\n"
+    "    _cg[_index(_gyo, _gxo)] = make_int3(new_r, new_g, new_b); }")
 
 
 class _Instances:
@@ -899,8 +944,7 @@ class FusedEngine:
         key = (src, dst, colors, steps, fast)
         launch = self._launches.get(key)
         if launch is None:
-            grid = _persistent_grid(self._inst.n_tiles,
-                                    self.program.lds_bytes)
+            grid = _persistent_grid(self._inst.n_tiles, function)
             launch = function.prepare(
                 np.uint64(src), np.uint64(dst), np.uint64(colors),
                 np.uint64(self._inst.table_buf.ptr),
@@ -1008,9 +1052,9 @@ class Batch:
         self.stream.synchronize()
 
     def _launch(self, steps, fast):
-        grid = _persistent_grid(self.inst.n_tiles, self.program.lds_bytes)
         function = (self.program.function_fast if fast
                     else self.program.function)
+        grid = _persistent_grid(self.inst.n_tiles, function)
         function(
             np.uint64(self.state[self.cur].ptr),
             np.uint64(self.state[1 - self.cur].ptr),

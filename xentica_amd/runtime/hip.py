@@ -64,6 +64,9 @@ _SIGNATURES = {
     'xen_launch': [_vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
                    ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
                    ctypes.c_uint, _vp, ctypes.POINTER(_vp)],
+    'xen_occupancy': [_vp, ctypes.c_uint, ctypes.c_uint,
+                      ctypes.POINTER(ctypes.c_int),
+                      ctypes.POINTER(ctypes.c_int)],
     'xen_malloc': [ctypes.POINTER(_vp), ctypes.c_size_t],
     'xen_free': [_vp],
     'xen_memcpy_h2d': [_vp, _vp, ctypes.c_size_t, _vp],
@@ -264,6 +267,20 @@ class Function:
             block[2], shared, stream.handle if stream else None, ptrs),
             "launching %s" % self.name)
 
+
+    def occupancy(self, block=256, shared=0):
+        """``(workgroups one CU holds at once, CUs)`` for ``block``-thread
+        workgroups of this kernel (registers and LDS counted)."""
+        # kept by the module (functions are made afresh by each lookup)
+        cache = self.module.__dict__.setdefault("_occupancy", {})
+        key = (self.name, block, shared)
+        if key not in cache:
+            per_cu, cus = ctypes.c_int(), ctypes.c_int()
+            check(library().xen_occupancy(
+                self.handle, block, shared, ctypes.byref(per_cu),
+                ctypes.byref(cus)), "occupancy of %s" % self.name)
+            cache[key] = (per_cu.value, cus.value)
+        return cache[key]
 
     def prepare(self, *args, grid=(1, 1, 1), block=(256, 1, 1), shared=0,
                 stream=None):
