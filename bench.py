@@ -316,6 +316,11 @@ def batched(steps=1000, plain_steps=10, tiles=None, depths=None):
         "cells": total, "value": value, "unit": "cell-updates/s",
         "ms": ms, "tile": list(batch.program.tile),
         "depth": batch.program.depth,
+        "algorithmic_bytes_per_cell_update": 2.0,
+        "algorithmic_GBps": value * 2.0 / 1e9,
+        "roofline_frac": value * 2.0 / 1e9 / HBM_PEAK_GBS,
+        "colour_on_last_step_only": bool(
+            batch.program.admits(batch.color_bounds)),
         "autotune": {"%dx%d/d%d" % (t[0], t[1], d): v
                      for (t, d), v in tuned.items()},
         "plain_loop_value": plain_value, "plain_loop_steps": plain_steps,
