@@ -105,3 +105,23 @@ def test_color_guard_only_for_pure_colors():
     noise = FusedProgram(NoiseTV(NoiseTVExperiment))
     assert not noise.color_guard and "if (_inner" not in noise.source
     assert noise.color_plan is None and noise.source_fast is None
+
+
+def test_direct_colour_store_only_where_it_raises_occupancy():
+    from examples.workloads import LifeLikeRule, batch_experiments
+    model = LifeLikeRule(batch_experiments(count=1)[0])
+    # 64x32: 27 KB of colour staging holds a CU to four workgroups; without
+    # it the planes need 7.7 KB and a CU holds eight
+    wide = FusedProgram(model, tile=(64, 32), depth=8)
+    assert wide.direct_colors
+    assert wide.lds_bytes_fast == 2 * 80 * 48 < wide.lds_bytes
+    assert "#define _CDIRECT 1" in wide.source_fast
+    assert "_cg[_index(_gyo, _gxo)] = make_int3(new_r, new_g, new_b);" \
+        in wide.source_fast
+    # the every-step form always stages the colours
+    assert "#define _CDIRECT 0" in wide.source
+    # 64x16 already fits eight workgroups with the staging
+    narrow = FusedProgram(model, tile=(64, 16), depth=8)
+    assert not narrow.direct_colors
+    assert narrow.lds_bytes_fast == narrow.lds_bytes
+    assert "#define _CDIRECT 0" in narrow.source_fast
