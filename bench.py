@@ -612,8 +612,31 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
     }
 
 
+def refbench(steps=100000, plain_steps=10000):
+    """
+    The reference's own benchmark (``tests/benchmark.py``: Conway's Life
+    640x360, Shifting Sands, Noise TV and EvoLife through ``model.step()``,
+    ``NUM_STEPS * cells_num / seconds``), on the default engine with the
+    reference's 100000 steps and on the plain engine with fewer.
+    """
+    from examples import benchmark
+    out = {"workload": "the reference's tests/benchmark.py models through "
+                       "model.step(); %d steps (plain engine %d)"
+                       % (steps, plain_steps)}
+    for engine, count in ((None, steps), ("plain", plain_steps)):
+        if not count:
+            continue
+        for name, (speed, cells, used) in benchmark.run(
+                count, engine, log=log).items():
+            out.setdefault(name, {"cells": cells})[used] = {
+                "value": speed, "unit": "cells/s",
+                "us_per_step": cells / speed * 1e6, "steps": count}
+    return out
+
+
 _LEGS = {
     "life512": life512,
+    "refbench": refbench,
     "render": render_leg,
     "life3d": life3d,
     "energy": energy_life,
@@ -641,6 +664,7 @@ def main():
     parser.add_argument("--no-surrogate", action="store_true")
     parser.add_argument("--no-512", action="store_true")
     parser.add_argument("--no-render", action="store_true")
+    parser.add_argument("--no-refbench", action="store_true")
     parser.add_argument("--leg-steps", type=int, default=0,
                         help="steps of the workload with --only life3d, "
                              "energy, batch or life512 (0 = the leg's "
@@ -742,6 +766,9 @@ def main():
     if not args.no_render:
         log("render ...")
         result["render"] = render_leg()
+    if not args.no_refbench:
+        log("the reference's benchmark models ...")
+        result["refbench"] = refbench()
     if not args.no_3d:
         log("3-D workload ...")
         result["life3d"] = life3d(cpu=not args.no_cpu)

@@ -40,3 +40,16 @@ def test_total_energy_conserved(cls, exp, bits, steps):
         moved = moved or not np.array_equal(now, start)
     assert moved, "the energy never moved"
     assert model.timestep == steps
+
+
+def test_reference_benchmark_script_runs():
+    """The reference's benchmark (examples/benchmark.py) on both engines:
+    every model reports a rate, and the default engine is the fused one
+    for the four models that qualify."""
+    from examples import benchmark
+    for engine in (None, "plain"):
+        results = benchmark.run(num_steps=5, engine=engine, log=None)
+        assert list(results) == [name for name, _, _ in benchmark.MODELS]
+        for speed, cells, used in results.values():
+            assert speed > 0 and cells == 640 * 360
+            assert used == (engine or "fused")
