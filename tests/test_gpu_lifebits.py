@@ -132,3 +132,31 @@ def test_life16k_vs_plain_engine():
         plain = kernels.crc32_u8(model.gpu.arrays.cells.device_ptr(),
                                  model.cells_num)
         assert life.crc32() == plain, "after %d steps" % (50 * (block + 1))
+
+
+def test_life16k_2000_steps_vs_plain_engine():
+    """
+    The headline configuration (depth 8, the library's strip count) over
+    the whole 2000 steps: the first 64 steps one launch each with the CRC
+    of every step, the rest in 8-step launches with the CRC after each,
+    all equal to the plain engine's (the reference's structure).
+    """
+    from examples.game_of_life import GameOfLife
+    from examples.workloads import Life16kExperiment
+    class PlainLife16k(Life16kExperiment):
+        engine = "plain"
+    model = GameOfLife(PlainLife16k)
+    cells = model.gpu.arrays.cells.get()[:model.cells_num]
+    life = LifeBitsField(16384, 16384, 8, 12, depth=8)
+    life.set_cells(cells)
+    done = 0
+    while done < 2000:
+        count = 1 if done < 64 else 8
+        for _ in range(count):
+            model.step()
+        life.run(count)
+        life.synchronize()
+        done += count
+        plain = kernels.crc32_u8(model.gpu.arrays.cells.device_ptr(),
+                                 model.cells_num)
+        assert life.crc32() == plain, "after %d steps" % done
