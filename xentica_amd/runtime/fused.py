@@ -88,11 +88,11 @@ class FusedProgram:
     #: bytes and for wider ones; the first that fits the LDS budget wins.
     #: The orders follow what the autotuner measured on MI355X: byte-sized
     #: Life-like states run fastest on 64x16 tiles at 6-8 steps per launch,
-    #: 16-byte words (EnergyLife) on 32x16 or 16x16 tiles at 2-3.
+    #: 8- and 16-byte words (EvoLife, EnergyLife) on 32x16 tiles at 2.
     CANDIDATES = (((64, 16), 8), ((64, 16), 6), ((32, 32), 4), ((32, 16), 4),
                   ((32, 32), 2), ((32, 16), 2), ((16, 16), 2), ((16, 16), 1),
                   ((8, 8), 1))
-    CANDIDATES_WIDE = (((32, 16), 3), ((16, 16), 2), ((32, 16), 2),
+    CANDIDATES_WIDE = (((32, 16), 2), ((32, 16), 3), ((16, 16), 2),
                        ((16, 16), 1), ((8, 8), 1))
 
     def __init__(self, model, tile=None, depth=None, fold=True, split=True):
