@@ -126,6 +126,10 @@ def _expr(rng, depth):
                                rng.randint(0, 7))
     if pick < 0.58:
         return "((unsigned int) ((%s * 0.5f)))" % _expr(rng, depth - 1)
+    if pick < 0.64:
+        return "(((float) (%s)) / %s)" % (
+            _expr(rng, depth - 1),
+            rng.choice(["65535", "360", "3", "7.0f", "((float) (18))"]))
     op = rng.choice(["+", "-", "*", "&", "|", "^", "<", "==", ">=", "&&",
                      "||"])
     return "(%s %s %s)" % (_expr(rng, depth - 1), op, _expr(rng, depth - 1))
@@ -282,7 +286,10 @@ def test_selects_equal_products_against_gcc(tmp_path):
             continue
         cases.append((types, spans, text, out))
     assert sum(" ? (" in out for *_, out in cases) > 100
-    body = ["#include <stdio.h>", "int main(void) {"]
+    assert sum("_xen_fdiv(" in out for *_, out in cases) > 20
+    body = ["#include <stdio.h>", "#include <math.h>",
+            "#define __device__", "#define __forceinline__ static inline",
+            ranges.FDIV_HELPER, "int main(void) {"]
     for idx, (types, spans, text, out) in enumerate(cases):
         (tx, ty, tz), ((xl, xh), (yl, yh), (zl, zh)) = types, spans
         body.append(
@@ -303,8 +310,8 @@ def test_selects_equal_products_against_gcc(tmp_path):
     src = tmp_path / "selects.c"
     src.write_text("\n".join(body))
     exe = tmp_path / "selects"
-    subprocess.run(["gcc", "-O0", "-fwrapv", "-w", "-o", str(exe), str(src)],
-                   check=True)
+    subprocess.run(["gcc", "-O0", "-fwrapv", "-w", "-o", str(exe), str(src),
+                    "-lm"], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True,
                          text=True).stdout.split()
     results = dict(zip(out[::2], out[1::2]))
@@ -331,3 +338,49 @@ def test_rewrite_statements():
     out, complete = ranges.Analysis({}).rewrite(
         "int a = 1;\nif (a) { a = 2; }\nint b = (a * 3);\n")
     assert not complete and out.endswith("int b = (a * 3);\n")
+
+
+def _rn32(value):
+    """A Fraction rounded to the nearest float32 (ties to even)."""
+    from fractions import Fraction
+    if value == 0:
+        return Fraction(0)
+    sign = -1 if value < 0 else 1
+    mag = abs(value)
+    exp = mag.numerator.bit_length() - mag.denominator.bit_length()
+    if Fraction(2) ** exp > mag:
+        exp -= 1
+    ulp = Fraction(2) ** (max(exp, -126) - 23)
+    whole, rem = divmod((mag / ulp).numerator, (mag / ulp).denominator)
+    twice, den = 2 * rem, (mag / ulp).denominator
+    if twice > den or (twice == den and whole % 2):
+        whole += 1
+    return sign * whole * ulp
+
+
+def _fdiv_exact_slow(c, lo, hi, recip):
+    """The division check with exact rational arithmetic throughout."""
+    from fractions import Fraction
+    c = _rn32(Fraction(c))
+    r = Fraction(recip)
+    for x in range(lo, hi + 1):
+        q = _rn32(x * r)
+        got = _rn32(_rn32(x - q * c) * r + q)
+        if got != _rn32(Fraction(x) / c):
+            return False
+    return True
+
+
+@pytest.mark.parametrize("c, lo, hi", [(65535, 0, 4095), (360, 0, 4095),
+                                       (3, -2048, 2047), (7, 0, 4095),
+                                       (0.1, 0, 1023)])
+def test_division_check_against_exact_arithmetic(c, lo, hi):
+    """The vectorised check of _xen_fdiv agrees with exact rational
+    arithmetic: it passes the rounded reciprocal (one fused correction
+    absorbs even an ulp of error in it) and fails one 2^-12 off."""
+    ok, recip = ranges.division_by_constant_exact(c, lo, hi)
+    assert ok and _fdiv_exact_slow(c, lo, hi, recip)
+    r32 = np.float32(1.0) / np.float32(c)
+    off = float(r32 * np.float32(1 + 2.0 ** -12))
+    bad, _ = ranges.division_by_constant_exact(c, lo, hi, recip=off)
+    assert not bad and not _fdiv_exact_slow(c, lo, hi, off)

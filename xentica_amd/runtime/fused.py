@@ -278,7 +278,7 @@ class FusedProgram:
             k=0 if self.folded else self.k, nt=BLOCK,
             rows=self.rows_per_thread, run=self.ring_run,
             he=self.halo_per_step, ctype=self.ctype, static=border,
-            cfast=cfast, cdirect=int(cdirect),
+            cfast=cfast, cdirect=int(cdirect), helpers=ranges.FDIV_HELPER,
             bvalue="%dULL" % (self.border_value & 0xFFFFFFFFFFFFFFFF),
             params_struct=params_struct, param_locals=param_locals,
             zero_slots=zero_slots, tb_decl=tb_decl, step_loop=step_loop)
@@ -346,7 +346,7 @@ _KERNEL_TEMPLATE = r"""
 // 1: only a launch's last step computes the colour, and the stored
 // colours are not read (launched only where the colour written provably
 // equals the new value at every step); 0: every step computes it
-#define _CFAST {cfast}
+{helpers}#define _CFAST {cfast}
 // 1: the last step stores the colours straight to HBM (no LDS staging)
 #define _CDIRECT {cdirect}
 typedef {ctype} _word;

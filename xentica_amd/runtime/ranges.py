@@ -96,12 +96,14 @@ _FLOAT_MAX = 3.4e38
 class Value:
     """A closed interval ``[lo, hi]`` of values of C type ``ctype``."""
 
-    __slots__ = ('ctype', 'lo', 'hi')
+    __slots__ = ('ctype', 'lo', 'hi', 'integral')
 
-    def __init__(self, ctype, lo, hi):
+    def __init__(self, ctype, lo, hi, integral=False):
         self.ctype = ctype
         self.lo = lo
         self.hi = hi
+        #: a float that holds an integer of [lo, hi] exactly
+        self.integral = integral
 
     @classmethod
     def full(cls, ctype):
@@ -168,7 +170,7 @@ def convert(val, ctype):
             return _fit(ctype, val.lo, val.hi)
         lo, hi = float(val.lo), float(val.hi)
         if max(abs(val.lo), abs(val.hi)) < (1 << 24):
-            return _fit(ctype, lo, hi)
+            return Value(ctype, lo, hi, integral=True)
         return _widen(ctype, lo, hi)
     if src.floating:
         if math.isinf(val.lo) or math.isinf(val.hi):
@@ -457,6 +459,9 @@ class _Parser:
             if op == "*" and self.selects:
                 text = _select_product(left, ltext, right, rtext,
                                        value.ctype) or text
+            elif op == "/" and self.selects:
+                text = _constant_division(left, ltext, right,
+                                          value.ctype) or text
             left, ltext = value, text
 
     def cast_type(self):
@@ -542,6 +547,8 @@ class _Parser:
         if kind == "float":
             ctype = _FLOAT if text[-1] in "fF" else _DOUBLE
             value = float(text.rstrip("fF"))
+            if ctype is _FLOAT and _is_float32(value):
+                return Value(ctype, value, value), text
             return _widen(ctype, value, value), text
         if kind == "op" and text == "(":
             value, inner = self.expression()
@@ -574,6 +581,87 @@ def _select_product(left, ltext, right, rtext, ctype):
             return "(%s ? (%s) %s : (%s) 0)" % (ctext, ctype.name, other,
                                                ctype.name)
     return None
+
+
+#: a float divided by a constant: a product with the rounded reciprocal and
+#: one fused correction, x * r + fma(x - x * r * c, r), for float kernels
+FDIV_HELPER = (
+    "__device__ __forceinline__ float _xen_fdiv(float x, float c, float r) {\n"
+    "    const float q = x * r;\n"
+    "    return __builtin_fmaf(__builtin_fmaf(-q, c, x), r, q);\n"
+    "}\n")
+
+_EXACT_DIVISIONS = {}
+
+
+def division_by_constant_exact(c, lo, hi, recip=None):
+    """
+    Whether ``_xen_fdiv(x, c, RN(1/c))`` is the correctly rounded float
+    quotient ``x / c`` for every integer ``x`` in ``[lo, hi]`` (float32,
+    round to nearest even), checked on every one of them.
+
+    The quotient is the hardware's IEEE division (NumPy).  The product
+    ``q = RN(x * r)`` is NumPy's float32 product; the residual
+    ``x - q * c`` is exact in float64 (``q * c`` has 48 significant bits
+    and lies within a factor 2 of ``x``) and the first fused operation
+    rounds it once; ``e * r`` is exact in float64 and the second fused
+    operation's sum is carried as a float64 pair (TwoSum), which rounds
+    to float32 as its leading part does unless that part is a float32
+    midpoint, where the trailing part's sign decides.
+    """
+    key = (float(c), int(lo), int(hi))
+    if recip is None and key in _EXACT_DIVISIONS:
+        return _EXACT_DIVISIONS[key]
+    import numpy as np
+    c32 = np.float32(c)
+    r32 = np.float32(np.float64(1.0) / np.float64(c32)) if recip is None \
+        else np.float32(recip)
+    ok = bool(np.isfinite(r32)) and float(c32) != 0.0
+    if ok and hi >= lo:
+        x = np.arange(lo, hi + 1, dtype=np.float64).astype(np.float32)
+        want = x / c32
+        q = x * r32
+        e = (x.astype(np.float64)
+             - q.astype(np.float64) * np.float64(c32)).astype(np.float32)
+        a = q.astype(np.float64)
+        b = e.astype(np.float64) * np.float64(r32)
+        s = a + b
+        bb = s - a
+        err = (a - (s - bb)) + (b - bb)
+        got = s.astype(np.float32)
+        # s exactly halfway between two floats: the trailing part decides
+        up = np.nextafter(got, np.float32(np.inf))
+        down = np.nextafter(got, np.float32(-np.inf))
+        half_up = (up.astype(np.float64) + got.astype(np.float64)) / 2
+        half_down = (down.astype(np.float64) + got.astype(np.float64)) / 2
+        got = np.where((s == half_up) & (err > 0), up, got)
+        got = np.where((s == half_down) & (err < 0), down, got)
+        ok = bool(np.array_equal(got, want))
+    result = (ok, float(r32)) if ok else (False, None)
+    if recip is None:
+        _EXACT_DIVISIONS[key] = result
+    return result
+
+
+def _constant_division(left, ltext, right, ctype):
+    """``left / right`` through ``_xen_fdiv`` where that is exact: a float
+    holding an integer of a range of at most 2^20 values, divided by a
+    float constant."""
+    exact_int = (not left.ctype.floating
+                 and max(abs(left.lo), abs(left.hi)) < (1 << 24))
+    if (ctype.name != "float" or not (left.integral or exact_int)
+            or right.lo != right.hi or left.hi - left.lo > (1 << 20)):
+        return None
+    ok, recip = division_by_constant_exact(right.lo, int(left.lo),
+                                           int(left.hi))
+    if not ok:
+        return None
+    return "_xen_fdiv(%s, %.9ef, %.9ef)" % (ltext, right.lo, recip)
+
+
+def _is_float32(value):
+    import struct
+    return struct.unpack("f", struct.pack("f", value))[0] == value
 
 
 def _is_pointer(value):
