@@ -331,25 +331,67 @@ def test_batch_graph_replays():
             assert crc(a) == crc(b)
 
 
-def test_batch_full_workload_subset_vs_plain():
-    """The 256-instance batch, 1000 steps; 24 instances (the largest and a
-    random draw) checked against the plain engine run alone."""
+def test_batch_full_workload_vs_plain_and_oracle():
+    """The 256-instance batch, 1000 steps, at the launch the autotuner picks
+    for it on MI355X (64x32 tiles, depth 8, graph): every instance equal to
+    the plain
+    engine run alone with its own rule, and the eight smallest also equal
+    to the oracle (the reference's plain structure on the CPU) with their
+    own birth/survival masks."""
+    from oracle import oracle as O
+    from xentica_amd.runtime import kernels
     exps = batch_experiments()
     models = [LifeLikeRule(exp) for exp in exps]
-    batch = Batch(models)
+    starts = [m.gpu.arrays.cells.get()[:m.cells_num] for m in models]
+    batch = Batch(models, tile=(64, 32), depth=8)
+    print("batch of %d, 1000 steps" % len(models), flush=True)
     batch.run(1000)
     batch.synchronize()
     batch.scatter()
-    sizes = [exp.size[0] for exp in exps]
-    rng = np.random.default_rng(11)
-    picks = {int(np.argmax(sizes)), int(np.argmin(sizes))}
-    picks |= set(int(x) for x in rng.choice(len(exps), 22, replace=False))
-    for idx in sorted(picks):
-        plain_exp = type("Plain%d" % idx, (exps[idx],), {"engine": "plain"})
+    got = [kernels.crc32_u8(m.gpu.arrays.cells.device_ptr(), m.cells_num)
+           for m in models]
+    for idx, exp in enumerate(exps):
+        plain_exp = type("Plain%d" % idx, (exp,), {"engine": "plain"})
         single = LifeLikeRule(plain_exp)
-        single.rule = exps[idx].rule
+        single.rule = exp.rule
         run(single, 1000)
-        assert crc(models[idx]) == crc(single), "instance %d" % idx
+        want = kernels.crc32_u8(single.gpu.arrays.cells.device_ptr(),
+                                single.cells_num)
+        assert got[idx] == want, "instance %d vs plain" % idx
+    sizes = [exp.size[0] for exp in exps]
+    for idx in np.argsort(sizes, kind="stable")[:8]:
+        exp = exps[idx]
+        field = O.Field(exp.size, 8, np.uint8, starts[idx])
+        O.lifelike(field, O.moore_deltas(2), O.moore_rev(2),
+                   exp.rule & 0x1FF, (exp.rule >> 9) & 0x1FF, 1000,
+                   colors=False)
+        assert got[idx] == binascii.crc32(field.cells), \
+            "instance %d vs oracle" % idx
+
+
+def test_batch_parameter_change_while_a_replay_runs():
+    """A rule changed between two back-to-back runs, with no host sync in
+    between, on instances large enough that the first replay is still
+    running when the table is uploaded: the second run sees the new rule
+    and the first the old one (the upload is ordered on the batch's
+    stream)."""
+    exps = batch_experiments(count=3, seed=5, side_min=1900, side_max=2048)
+    models = [LifeLikeRule(exp) for exp in exps]
+    batch = Batch(models, tile=(64, 16), depth=8)
+    rules = [exp.rule for exp in exps]
+    new_rule = 6152 if rules[0] != 6152 else 0b111000 | (0b1100 << 9)
+    batch.run(96)
+    models[0].rule = new_rule
+    batch.run(96)
+    batch.synchronize()
+    batch.scatter()
+    for idx, (model, exp) in enumerate(zip(models, exps)):
+        plain = LifeLikeRule(type("P%d" % idx, (exp, ), {"engine": "plain"}))
+        plain.rule = exp.rule
+        run(plain, 96)
+        plain.rule = new_rule if idx == 0 else exp.rule
+        run(plain, 96)
+        assert crc(model) == crc(plain), idx
 
 
 @pytest.mark.parametrize("size", [(5, 3), (1, 7), (33, 2), (70, 65)])

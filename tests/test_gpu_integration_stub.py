@@ -37,7 +37,10 @@ def test_reference_side_stub_runs_generated_kernels():
                                     len(self.OPTS), ctypes.byref(code),
                                     ctypes.byref(size)))
             self.mod = _vp()
-            _check(_lib.xen_module_load(code, ctypes.byref(self.mod)))
+            try:  # the module keeps its own copy of the code object
+                _check(_lib.xen_module_load(code, ctypes.byref(self.mod)))
+            finally:
+                _lib.xen_code_free(code)
 
         def get_function(self, name):
             fn = _vp()
@@ -78,3 +81,4 @@ def test_reference_side_stub_runs_generated_kernels():
     assert np.array_equal(out[:n], model.gpu.arrays.cells.get()[:n])
     _check(_lib.xen_free(fld))
     _check(_lib.xen_free(col))
+    _check(_lib.xen_module_unload(module.mod))

@@ -134,12 +134,16 @@ def test_life16k_vs_plain_engine():
         assert life.crc32() == plain, "after %d steps" % (50 * (block + 1))
 
 
-def test_life16k_2000_steps_vs_plain_engine():
+@pytest.mark.parametrize("strips", [341, 455])
+def test_life16k_2000_steps_vs_plain_engine(strips):
     """
-    The headline configuration (depth 8, the library's strip count) over
-    the whole 2000 steps: the first 64 steps one launch each with the CRC
-    of every step, the rest in 8-step launches with the CRC after each,
-    all equal to the plain engine's (the reference's structure).
+    The headline configuration over the whole 2000 steps, at the launch
+    shape the benchmark times (depth 8; 341 row strips, three waves per
+    SIMD, the library's default and the autotuner's usual choice on 256
+    CUs; and 455, four waves per SIMD): the first 64 steps one launch each
+    with the CRC of every step, the rest in 8-step launches with the CRC
+    after each, all equal to the plain engine's (the reference's
+    structure).
     """
     from examples.game_of_life import GameOfLife
     from examples.workloads import Life16kExperiment
@@ -147,8 +151,9 @@ def test_life16k_2000_steps_vs_plain_engine():
         engine = "plain"
     model = GameOfLife(PlainLife16k)
     cells = model.gpu.arrays.cells.get()[:model.cells_num]
-    life = LifeBitsField(16384, 16384, 8, 12, depth=8)
+    life = LifeBitsField(16384, 16384, 8, 12, depth=8, strips=strips)
     life.set_cells(cells)
+    print("Life 16384^2, depth 8, %d strips, 2000 steps" % strips, flush=True)
     done = 0
     while done < 2000:
         count = 1 if done < 64 else 8

@@ -172,3 +172,42 @@ def test_training_beats_persistence():
     assert losses[-1] < 0.01 * losses[0]
     assert after["l1_relative"] <= 0.02
     assert after["l1"] < after["persistence_l1"] < before["l1"]
+
+
+def test_predict_and_forward_at_the_workload_size():
+    """
+    The surrogate at its workload's size: a 4096x4096 field of sand heights
+    0-8 coarse-grained to 256x256 cells (the predict launch's real shape,
+    512 workgroups of 8x16 coarse cells).  predict equals encode + forward
+    bit for bit, the encoder equals the oracle's, and the forward is
+    within ATOL_Y of the oracle on every one of the 2^20 outputs.
+    """
+    rng = np.random.default_rng(4096)
+    fine = rng.integers(0, 9, (1, 4096, 4096), dtype=np.uint8)
+    h = w = 256
+    dev = torch.from_numpy(fine).cuda()
+    net = NCASurrogate(seed=21)
+    net.set_normalization(4.0, 2.5)
+    params = net.inference_params()
+    print("surrogate 4096^2 -> 256^2", flush=True)
+    x = torch.empty((1, h, w, 16), device="cuda")
+    mean = torch.empty((1, h, w), device="cuda")
+    y_ref = torch.empty_like(x)
+    pred_ref = torch.empty((1, h, w), device="cuda")
+    network.encode(dev.data_ptr(), 1, h, w, x, mean, value_mask=0x0F)
+    network.forward_raw(x, params, y_ref, pred_ref)
+    y = torch.full_like(x, float("nan"))
+    pred = torch.full_like(pred_ref, float("nan"))
+    network.predict(dev.data_ptr(), 1, h, w, params, y, pred,
+                    value_mask=0x0F)
+    torch.cuda.synchronize()
+    assert torch.equal(y, y_ref)
+    assert torch.equal(pred, pred_ref)
+    want_x, want_mean = nca.encode(fine, mask=0x0F)
+    assert np.array_equal(x.cpu().numpy(), want_x.astype(np.float32))
+    assert np.array_equal(mean.cpu().numpy(), want_mean.astype(np.float32))
+    oracle_params = {name: tensor.float().cpu().numpy()
+                     for (name, _), tensor in zip(nca.PARAM_SHAPES, params)}
+    want_y, _ = nca.forward(x.cpu().numpy(), oracle_params, rounding=True)
+    assert np.abs(y.cpu().numpy() - want_y).max() <= ATOL_Y
+    assert np.abs(pred.cpu().numpy() - want_y.mean(axis=-1)).max() <= ATOL_Y

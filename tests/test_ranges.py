@@ -384,3 +384,17 @@ def test_division_check_against_exact_arithmetic(c, lo, hi):
     off = float(r32 * np.float32(1 + 2.0 ** -12))
     bad, _ = ranges.division_by_constant_exact(c, lo, hi, recip=off)
     assert not bad and not _fdiv_exact_slow(c, lo, hi, off)
+
+
+def test_division_emits_the_divisor_it_checked():
+    """A divisor that float32 cannot hold is written into the kernel as the
+    float32 value the exactness check used, not as its decimal."""
+    flt = ranges.CTYPES[("float", )]
+    left = ranges.Value(flt, 0, 4095, integral=True)
+    c = 16777217.0  # 2^24 + 1 rounds to 2^24 in float32
+    right = ranges.Value(flt, c, c)
+    text = ranges._constant_division(left, "x", right, flt)
+    assert text is not None
+    divisor = text.split(",")[1].strip()
+    assert np.float32(float(divisor.rstrip("f"))) == np.float32(c)
+    assert float(divisor.rstrip("f")) == float(np.float32(c))

@@ -876,15 +876,21 @@ class _Instances:
                                 for p in self.program.params))
         return values
 
-    def refresh(self, values=None):
-        """Upload the table when parameter values changed."""
+    def refresh(self, values=None, stream=None):
+        """Upload the table when parameter values changed.
+
+        The copy is queued on ``stream``, the stream the table's readers
+        run on, and waits for it: a launch still reading the old table
+        there finishes first (the null stream does not wait for the
+        non-blocking streams batches run on).
+        """
         if values is None:
             values = self.param_values()
         if values != self._uploaded:
             for idx, vals in enumerate(values):
                 for param, val in zip(self.program.params, vals):
                     self.table[idx]['p_' + param.name] = val
-            self.table_buf.upload(self.table)
+            self.table_buf.upload(self.table, stream)
             self._uploaded = values
 
 
@@ -1019,9 +1025,14 @@ class Batch:
         """Copy every model's current state into the batch buffers."""
         from xentica_amd.runtime.hip import check, library
         lib = library()
+        for model in self.models:
+            model._engine.flush()
+        # the flushes queue kernels and copies on the null stream, which
+        # the batch's non-blocking stream does not wait for; the batch's
+        # own earlier launches read the buffers the copies overwrite
+        hip.device().synchronize()
         for model, off, coff in zip(self.models, self.inst.offsets,
                                     self.inst.col_offsets):
-            model._engine.flush()
             arrays = model.gpu.arrays
             check(lib.xen_memcpy_d2d(
                 self.state[self.cur].ptr + off * self.inst.itemsize,
@@ -1071,8 +1082,9 @@ class Batch:
                                               else [])
 
     def run(self, steps, graph=True):
-        """Advance every instance ``steps`` steps (asynchronous)."""
-        self.inst.refresh()
+        """Advance every instance ``steps`` steps (asynchronous; a change
+        of parameter values waits for the batch's earlier launches)."""
+        self.inst.refresh(stream=self.stream)
         fast, function = self.program.kernel_for(self.color_bounds)
         plan = self.launch_plan(steps)
         if not plan:

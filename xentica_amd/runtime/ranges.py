@@ -656,7 +656,11 @@ def _constant_division(left, ltext, right, ctype):
                                            int(left.hi))
     if not ok:
         return None
-    return "_xen_fdiv(%s, %.9ef, %.9ef)" % (ltext, right.lo, recip)
+    # the divisor the check used: the constant rounded to float32, written
+    # with enough digits to parse back to that same float
+    import numpy as np
+    divisor = float(np.float32(right.lo))
+    return "_xen_fdiv(%s, %.9ef, %.9ef)" % (ltext, divisor, recip)
 
 
 def _is_float32(value):
