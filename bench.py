@@ -298,6 +298,8 @@ def batched(steps=1000, plain_steps=10, tiles=None, depths=None):
             plain_exp = type(exp.__name__ + "Plain", (exp,),
                              {"engine": "plain"})
             plain_models.append(LifeLikeRule(plain_exp))
+        hip.precompile([model.cuda_source for model in plain_models],
+                       hip.compile_options(LifeLikeRule.fp_contract))
         for model, exp in zip(plain_models, exps):
             model.rule = exp.rule
             model.step()

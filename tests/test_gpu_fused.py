@@ -19,6 +19,7 @@ from examples.evolife import (
 from examples.workloads import (
     Life512Experiment, LifeLikeRule, batch_experiments)
 from xentica_amd.core import color_effects
+from xentica_amd.runtime import hip
 from xentica_amd.runtime.fused import Batch, FusedEngine, TUNE_TILES
 
 pytestmark = pytest.mark.gpu
@@ -350,9 +351,12 @@ def test_batch_full_workload_vs_plain_and_oracle():
     batch.scatter()
     got = [kernels.crc32_u8(m.gpu.arrays.cells.device_ptr(), m.cells_num)
            for m in models]
-    for idx, exp in enumerate(exps):
-        plain_exp = type("Plain%d" % idx, (exp,), {"engine": "plain"})
-        single = LifeLikeRule(plain_exp)
+    singles = [LifeLikeRule(type("Plain%d" % idx, (exp,), {"engine": "plain"}))
+               for idx, exp in enumerate(exps)]
+    # one plain module per field size: compiled in worker processes first
+    hip.precompile([m.cuda_source for m in singles],
+                   hip.compile_options(LifeLikeRule.fp_contract))
+    for idx, (exp, single) in enumerate(zip(exps, singles)):
         single.rule = exp.rule
         run(single, 1000)
         want = kernels.crc32_u8(single.gpu.arrays.cells.device_ptr(),

@@ -220,6 +220,49 @@ def compile_source(source, name="xentica.hip", options=COMPILE_OPTIONS):
     return blob
 
 
+def _compile_into_cache(source, options):
+    """Worker of :func:`precompile`: compile one source into the cache."""
+    compile_source(source, "precompiled.hip", tuple(options))
+    return True
+
+
+def precompile(sources, options=COMPILE_OPTIONS, workers=0):
+    """
+    Compile many sources into the on-disk code-object cache at once.
+
+    hiprtc serialises compilations within one process, so the sources go
+    to worker processes (``workers``, default: the usable cores, at most
+    16); the workers only compile and never touch the GPU.  Sources
+    already cached are skipped.  Later :func:`compile_source` calls of
+    the same source and options read the cache.
+    """
+    options = tuple(options)
+    todo = []
+    seen = set()
+    for source in sources:
+        key = hashlib.sha256(
+            ("\0".join(options) + "\0" + source).encode()).hexdigest()[:32]
+        if key in seen or os.path.exists(
+                os.path.join(_cache_dir(), key + ".co")):
+            continue
+        seen.add(key)
+        todo.append(source)
+    if not todo:
+        return 0
+    if len(todo) == 1:
+        compile_source(todo[0], "precompiled.hip", options)
+        return 1
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+    workers = workers or min(16, len(os.sched_getaffinity(0)))
+    workers = max(1, min(workers, len(todo)))
+    with ProcessPoolExecutor(
+            workers, mp_context=multiprocessing.get_context("spawn")) as pool:
+        list(pool.map(_compile_into_cache, todo, [options] * len(todo),
+                      chunksize=max(1, len(todo) // (4 * workers))))
+    return len(todo)
+
+
 def _as_ctype(arg):
     """Convert one kernel argument to a ctypes object."""
     if hasattr(arg, "device_ptr"):
