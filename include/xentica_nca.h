@@ -65,6 +65,22 @@ int xen_nca_backward(const float *x, const float *dy, long long batch, int h, in
                      const uint16_t *w2, const float *b2, float *partials,
                      long long partial_rows, float *grads, xen_stream stream);
 
+/* xen_nca_backward without the caller's workspace and without allocating
+ * anything: the partial rows live in a static device array of the library
+ * (256 rows of XEN_NCA_PARAMS floats, 2.3 MB per GPU, held until the
+ * process ends; a grid that would need more rows is rejected, which no
+ * grid does today since the backward launch caps its workgroups at 256).
+ * No counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  Every call on one GPU shares that GPU's
+ * workspace: calls that overlap on one GPU (different streams or host
+ * threads) race and must be serialised by the caller.  Results equal
+ * xen_nca_backward's bit for bit. */
+int xen_nca_backward_noalloc(const float *x, const float *dy, long long batch, int h, int w,
+                             const float *dw, const float *bdw, const uint16_t *w1,
+                             const float *b1, const uint16_t *w2, const float *b2,
+                             float *grads, xen_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

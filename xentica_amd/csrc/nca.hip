@@ -27,6 +27,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "xentica_hip.h"
 #include "xentica_nca.h"
 #include "common.h"
@@ -484,6 +486,37 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(const float *__restrict_
     if (slice == 0 && p < kParams) grads[p] = part[0][lp];
 }
 
+// Workspace of xen_nca_backward_noalloc: the partial-gradient rows of
+// backward_kernel (one per workgroup; launches never use more than
+// kScratchRows workgroups).  Every GPU the library runs on holds its own
+// copy (2.3 MB) until the process ends.  Host code must not pass the
+// array's name to a kernel: it stands for the host-side shadow, not a
+// device address; the device address comes from hipGetSymbolAddress.
+constexpr int kScratchRows = 256;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) float scratch[kScratchRows * kParams];
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the call also makes the module's globals resident there).
+int scratch_address(float **out) {
+    static std::mutex lock;
+    static float *addr[kMaxDevices] = {};
+    int dev = 0;
+    if (int rc = xen::check(hipGetDevice(&dev), "hipGetDevice")) return rc;
+    if (dev < 0 || dev >= kMaxDevices)
+        return xen::fail("xen_nca_backward_noalloc: device index out of range", 22);
+    std::lock_guard<std::mutex> guard(lock);
+    if (!addr[dev]) {
+        void *ptr = nullptr;
+        if (int rc = xen::check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)),
+                                "hipGetSymbolAddress(scratch)"))
+            return rc;
+        addr[dev] = static_cast<float *>(ptr);
+    }
+    *out = addr[dev];
+    return 0;
+}
+
 int check_grid(long long batch, int h, int w, long long *chunks) {
     if (batch < 1 || h < 1 || w < 1)
         return xen::fail("xen_nca: batch, h and w must be positive", 22);
@@ -571,6 +604,26 @@ int xen_nca_backward(const float *x, const float *dy, long long batch, int h, in
     hipLaunchKernelGGL(reduce_kernel, dim3((kParams + kRedParams - 1) / kRedParams),
                        dim3(kBlock), 0, s, partials, static_cast<long long>(groups), grads);
     return xen::check(hipGetLastError(), "reduce_kernel launch");
+}
+
+int xen_nca_backward_noalloc(const float *x, const float *dy, long long batch, int h, int w,
+                             const float *dw, const float *bdw, const uint16_t *w1,
+                             const float *b1, const uint16_t *w2, const float *b2,
+                             float *grads, xen_stream stream) {
+    long long chunks;
+    if (int rc = check_grid(batch, h, w, &chunks)) return rc;
+    const long long rows = xen_nca_partial_rows(batch, h, w);
+    if (rows < 1 || rows > kScratchRows)
+        return xen::fail("xen_nca_backward_noalloc: the grid needs " + std::to_string(rows) +
+                             " workspace rows, more than the " +
+                             std::to_string(kScratchRows) + " the static workspace holds",
+                         22);
+    float *partials = nullptr;
+    if (int rc = scratch_address(&partials)) return rc;
+    // every row the reduction reads is written in full by backward_kernel
+    // first, so the workspace needs no clearing between calls
+    return xen_nca_backward(x, dy, batch, h, w, dw, bdw, w1, b1, w2, b2, partials, rows, grads,
+                            stream);
 }
 
 }  // extern "C"
