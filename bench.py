@@ -1,24 +1,28 @@
 """
-Headline benchmark: Game of Life B3/S23 on a 16384x16384 torus.
+Headline benchmark: Game of Life B3/S23 on a 16384x16384 torus, through
+the reference's API.
 
-One "step" is one generation of the whole field.  The field is seeded as
-the Life experiment with word 42 (``RandInt(0, 1)`` on every cell, 50 %
-density), packed to 64 cells per uint64 in HBM, and advanced by the
-bit-packed engine (``xen_life_bits_run``: temporal depth K steps per
-launch).  Inputs are resident in HBM before the timed region; the timed
-region is exactly ``--steps`` generations bracketed by device
-synchronisation, timed with HIP events on the engine's stream.
+One "step" is one generation of the whole field: ``model.step()`` of
+``GameOfLife(Life16kExperiment)`` (word 42, ``RandInt(0, 1)`` on every
+cell, 50 % density) on the default engine, which runs this model on the
+bit-sliced kernel (``xen_life_bits_run``: 64 cells per uint64, 8 steps per
+launch in registers).  The state is resident in HBM before the timed
+region; the timed region is exactly ``--steps`` calls of ``model.step()``
+plus the launch of the last queued steps, between HIP events on the
+engine's stream (the null stream), bracketed by device synchronisation.
+The final state of the timed run is then compared with the plain engine
+(the reference's emit/absorb structure) after the same number of steps
+("verified"); a mismatch makes the run fail.
 
 The JSON line also carries:
 * ``roofline`` -- algorithmic bytes (one bit read + one bit written per
-  cell-update = 0.25 B) per launch over the average launch time, against
-  MI355X's 8 TB/s HBM peak; ``traffic`` is the HBM bytes per launch from
-  the committed rocprofv3 counter summary when it exists;
-* ``cpu_baseline`` -- the oracle (the reference's plain emit/absorb
-  structure in C, OpenMP over every host core) on a bounded sample of the
-  same workload;
-* ``plain_engine`` -- the reference-structured GPU path (generated
-  emit/absorb kernels with the buffer planes in HBM) on the same field.
+  cell-update = 0.25 B) over the timed region against MI355X's 8 TB/s HBM
+  peak; ``traffic`` is the HBM bytes of one 8-step launch from the
+  committed rocprofv3 counters, ``hbm_frac`` that traffic over the launch
+  time, ``valu_frac`` the VALU issue fraction from the committed counters;
+* ``cpu_baseline`` -- the same model's generated emit/absorb code compiled
+  for the host with OpenMP (``runtime/hostc.py``) on a bounded sample;
+* ``plain_engine`` -- the reference-structured GPU path on the same field.
 
 Usage: ``python bench.py [--gpus 1] [--steps 2000] [--warmup 200]``.
 """
@@ -47,100 +51,107 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def seed_cells():
-    """Step-0 cells of the Life experiment (word 42, RandInt(0, 1))."""
-    from xentica_amd.seeds.random import LocalRandom
-    n = SIZE[0] * SIZE[1]
-    return LocalRandom(WORD).numpy.randint(0, 2, n).astype(np.uint8)
-
-
-def timed(field, stream, steps):
-    """Run ``steps`` steps between HIP events on ``stream``; returns ms."""
-    from xentica_amd.runtime import hip
-    start, stop = hip.Event(), hip.Event()
-    stream.synchronize()
-    start.record(stream)
-    field.run(steps)
-    stop.record(stream)
-    stop.synchronize()
-    return start.elapsed_ms(stop)
-
-
-def autotune(field, stream, trial_steps=96):
+def headline(steps, warmup):
     """
-    Pick the temporal depth and the number of row strips (as waves per
-    SIMD of the device: taller strips repeat fewer halo rows, more strips
-    hide more latency) with the best throughput on this device.
+    ``steps`` calls of ``GameOfLife(Life16kExperiment).step()`` after
+    ``warmup`` untimed ones, between HIP events on the engine's stream;
+    returns ``(model, milliseconds)``.
     """
-    from xentica_amd.runtime import hip
-    simds = 4 * hip.device().compute_units
-    tiles_x = -(-(field.width // 32) // 62)
-    best = None
-    for depth in (5, 6, 8):
-        for waves in (2, 3, 4, 6, 8):
-            # rounded down: one wave more than waves * simds puts a whole
-            # extra workgroup on one CU, which then finishes last
-            strips = max(1, min(field.height, waves * simds // tiles_x))
-            field.depth, field.strips = depth, strips
-            timed(field, stream, 2 * depth)
-            ms = timed(field, stream, trial_steps)
-            log("autotune depth=%d strips=%d: %.3f ms / %d steps"
-                % (depth, strips, ms, trial_steps))
-            if best is None or ms < best[2]:
-                best = (depth, strips, ms)
-    field.depth, field.strips = best[0], best[1]
-    return best[0]
-
-
-def traffic_per_launch():
-    """HBM bytes per launch from the committed counter summary, if any."""
-    if not os.path.exists(TRAFFIC_FILE):
-        return None
-    with open(TRAFFIC_FILE) as summary:
-        return json.load(summary).get("hbm_bytes_per_launch")
-
-
-def cpu_baseline(cells, steps=20):
-    """The oracle on a bounded sample: the same field, ``steps`` steps."""
-    from oracle import oracle
-    cores = len(os.sched_getaffinity(0))
-    field = oracle.Field(SIZE, 8, np.uint8, cells)
-    oracle.lifelike(field, oracle.moore_deltas(2), oracle.moore_rev(2), 8, 12,
-                    1, colors=False, threads=cores)
-    t0 = time.perf_counter()
-    oracle.lifelike(field, oracle.moore_deltas(2), oracle.moore_rev(2), 8, 12,
-                    steps, colors=False, threads=cores)
-    elapsed = time.perf_counter() - t0
-    return {
-        "value": SIZE[0] * SIZE[1] * steps / elapsed,
-        "unit": "cell-updates/s",
-        "cores": cores,
-        "kind": "port",
-        "sample": ("16384x16384 Life B3/S23, %d steps after 1 warm-up step, "
-                   "reference plain emit/absorb structure with 8 buffer "
-                   "planes (no colours), C + OpenMP" % steps),
-    }
-
-
-def plain_engine(steps=20):
-    """Cell-updates/s of the reference-structured GPU path on this field."""
     from examples.game_of_life import GameOfLife
     from examples.workloads import Life16kExperiment
     from xentica_amd.runtime import hip
-    class PlainLife16k(Life16kExperiment):
-        engine = "plain"
-    model = GameOfLife(PlainLife16k)
-    model.step()
+    model = GameOfLife(Life16kExperiment)
+    engine = model._engine
+    if engine.name != "bits":
+        raise SystemExit("the default engine of Life 16384^2 is %r, not the "
+                         "bit-sliced one" % engine.name)
+    for _ in range(warmup):
+        model.step()
+    engine.finish()
+    engine.state_ptr  # pylint: disable=pointless-statement
+    hip.device().synchronize()
+    start, stop = hip.Event(), hip.Event()
+    start.record(None)
+    for _ in range(steps):
+        model.step()
+    engine.finish()
+    stop.record(None)
+    stop.synchronize()
+    hip.device().synchronize()
+    return model, start.elapsed_ms(stop)
+
+
+def counters():
+    """The committed rocprofv3 counter summary of the headline kernel."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return {}
+    with open(TRAFFIC_FILE) as summary:
+        return json.load(summary)
+
+
+def host_baseline(model, steps, sample, warm=1):
+    """
+    The model's generated emit/absorb code compiled for the host with
+    OpenMP (``runtime/hostc.py``: the reference's plain structure, colours
+    included) on this machine's cores: ``steps`` steps after ``warm``
+    untimed ones, from the model's step-0 state.
+    """
+    from xentica_amd.runtime.hostc import PlainHost, host_threads
+    host = PlainHost(model)
+    host.load(model.gpu.arrays.cells.get(), model.gpu.arrays.colors.get())
+    cores = host_threads()
+    host.step(warm, threads=cores)
+    t0 = time.perf_counter()
+    host.step(steps, threads=cores)
+    elapsed = time.perf_counter() - t0
+    return {"value": model.cells_num * steps / elapsed,
+            "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": sample}
+
+
+def cpu_baseline(steps=4):
+    """The headline model's generated code on the host cores, ``steps``
+    steps of the same 16384^2 field."""
+    from examples.game_of_life import GameOfLife
+    from examples.workloads import Life16kExperiment
+    cpu_exp = type("Life16kHost", (Life16kExperiment, ), {"engine": "plain"})
+    return host_baseline(
+        GameOfLife(cpu_exp), steps,
+        "GameOfLife 16384x16384 word 42, %d steps after 1 warm-up step: the "
+        "DSL's generated emit/absorb (8 buffer planes, MovingAverage "
+        "colours) compiled for the host, g++ -O3 -fopenmp" % steps)
+
+
+def plain_reference(total_steps, timed_steps=20):
+    """
+    The plain engine (the reference's emit/absorb structure) on the
+    headline field: the device CRC after ``total_steps`` steps, and its
+    cell-updates/s over ``timed_steps`` of them.
+    """
+    from examples.game_of_life import GameOfLife
+    from examples.workloads import Life16kExperiment
+    from xentica_amd.runtime import hip, kernels
+    plain_exp = type("PlainLife16k", (Life16kExperiment, ),
+                     {"engine": "plain"})
+    model = GameOfLife(plain_exp)
+    timed_steps = max(0, min(timed_steps, total_steps - 1))
+    rest = total_steps - timed_steps
+    for _ in range(rest):
+        model.step()
     hip.device().synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for _ in range(timed_steps):
         model.step()
     hip.device().synchronize()
     elapsed = time.perf_counter() - t0
-    return {"value": model.cells_num * steps / elapsed,
-            "unit": "cell-updates/s", "steps": steps,
-            "structure": "generated emit + absorb, 8 buffer planes + colours "
-                         "in HBM, one launch pair per step"}
+    crc = kernels.crc32_u8(model.gpu.arrays.cells.device_ptr(),
+                           model.cells_num)
+    rate = {"value": model.cells_num * timed_steps / elapsed,
+            "unit": "cell-updates/s", "steps": timed_steps,
+            "structure": "generated emit + absorb, 8 buffer planes + "
+                         "colours in HBM, one launch pair per step"} \
+        if timed_steps else None
+    return crc, rate
 
 
 def life512(steps=1000, cpu=True):
@@ -644,7 +655,6 @@ _LEGS = {
     "energy": energy_life,
     "batch": batched,
     "surrogate": surrogate,
-    "plain": plain_engine,
 }
 
 
@@ -653,11 +663,6 @@ def main():
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=2000)
     parser.add_argument("--warmup", type=int, default=200)
-    parser.add_argument("--depth", type=int, default=0,
-                        help="temporal depth (0 = autotune)")
-    parser.add_argument("--strips", type=int, default=0,
-                        help="row strips with --depth (0 = the library's "
-                             "choice)")
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--no-plain", action="store_true")
     parser.add_argument("--no-batch", action="store_true")
@@ -703,31 +708,27 @@ def main():
         return 0
 
     from xentica_amd.runtime import hip
-    from xentica_amd.runtime.lifebits import LifeBitsField
 
     device = hip.device()
     log("device: %s" % device.name)
-    cells = seed_cells()
-    stream = hip.Stream()
-    field = LifeBitsField(SIZE[0], SIZE[1], born=8, sustain=12, stream=stream)
-    field.set_cells(cells)
-    depth = args.depth or autotune(field, stream)
-    field.depth = depth
-    if args.depth:
-        field.strips = args.strips
-    field.set_cells(cells)
-    log("depth %d; warmup %d steps" % (depth, args.warmup))
-    if args.warmup:
-        timed(field, stream, args.warmup)
-    ms = timed(field, stream, args.steps)
-    n = SIZE[0] * SIZE[1]
+    log("headline: GameOfLife 16384^2 through model.step(), warmup %d, "
+        "steps %d" % (args.warmup, args.steps))
+    model, ms = headline(args.steps, args.warmup)
+    engine = model._engine
+    n = model.cells_num
     value = n * args.steps / (ms / 1e3)
-    launches = -(-args.steps // depth)
-    launch_ms = ms / launches
-    bytes_per_launch = n * depth * BYTES_PER_CELL_UPDATE
-    achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9
     log("%d steps in %.3f ms: %.4g cell-updates/s" % (args.steps, ms, value))
-
+    crc = engine.crc32()
+    total = args.warmup + args.steps
+    plain_crc, plain_rate = plain_reference(
+        total, 0 if args.no_plain else 20)
+    verified = crc == plain_crc
+    log("verification against the plain engine after %d steps: %s"
+        % (total, verified))
+    achieved = value * BYTES_PER_CELL_UPDATE / 1e9
+    prof = counters()
+    traffic = prof.get("hbm_bytes_per_launch")
+    launch_s = ms / 1e3 / args.steps * engine.depth
     result = {
         "metric": METRIC,
         "value": value,
@@ -743,25 +744,34 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "life_b3s23_torus_16384x16384_bitpacked",
+            "api": "GameOfLife(Life16kExperiment).step(), default engine "
+                   "(bits)",
             "size": list(SIZE), "rule": "B3/S23", "seed_word": WORD,
-            "density": 0.5, "temporal_depth": depth,
-            "row_strips": field.strips,
+            "density": 0.5, "temporal_depth": engine.depth,
+            "row_strips": engine.strips or "library default (three waves "
+                                           "per SIMD)",
             "layout": "64 cells per uint64 (32 per uint32 lane word)",
             "device": device.name,
         },
+        "verified": verified,
+        "crc_final": crc,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_per_launch(),
+            "traffic": traffic,
+            "hbm_frac": (traffic / launch_s / 1e9 / HBM_PEAK_GBS
+                         if traffic else None),
+            "valu_frac": prof.get("valu_frac"),
+            "binding_resource": "VALU issue (temporal blocking: 8 steps per "
+                                "HBM round trip)",
         },
     }
-    if not args.no_plain:
-        log("plain engine ...")
-        result["plain_engine"] = plain_engine()
-        result["speedup_vs_plain"] = value / result["plain_engine"]["value"]
+    if plain_rate:
+        result["plain_engine"] = plain_rate
+        result["speedup_vs_plain"] = value / plain_rate["value"]
     if not args.no_512:
         log("Life 512^2 through model.step() ...")
         result["life512"] = life512(cpu=not args.no_cpu)
@@ -785,9 +795,9 @@ def main():
         result["surrogate"] = surrogate()
     if not args.no_cpu:
         log("cpu baseline ...")
-        result["cpu_baseline"] = cpu_baseline(cells)
+        result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)
-    return 0
+    return 0 if verified else 1
 
 
 if __name__ == "__main__":

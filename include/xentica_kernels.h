@@ -70,6 +70,17 @@ int xen_life3d_u8_direct_step(const void *src, void *dst, int width, int height,
 int xen_bits_pack(const uint8_t *cells, uint32_t *bits, long long n_cells, xen_stream stream);
 int xen_bits_unpack(const uint32_t *bits, uint8_t *cells, long long n_cells, xen_stream stream);
 
+/* A packed plane expanded into the reference's arrays in one pass: cells
+ * (may be NULL) receives one 0/1 byte per cell, colors (may be NULL) the
+ * int3 colour of each cell, color_dead[0..2] for dead and color_alive[0..2]
+ * for live cells (host pointers).  Replaces reading the main plane and the
+ * colours that absorb() wrote (xentica/core/base.py:659-661,
+ * xentica/core/color_effects.py:66-82) for a model whose colour is a
+ * function of its 1-bit state.  n_cells a multiple of 32; cells 4-byte and
+ * colors 16-byte aligned. */
+int xen_bits_expand(const uint32_t *bits, uint8_t *cells, int32_t *colors, long long n_cells,
+                    const int32_t *color_dead, const int32_t *color_alive, xen_stream stream);
+
 /* The low byte of each of n int32 values into dst: the
  * `img.get().astype(numpy.uint8)` of the reference's render()
  * (xentica/core/base.py:753) done on the device, so that a quarter of the

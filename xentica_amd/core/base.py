@@ -488,11 +488,12 @@ class CellularAutomaton(Translator, metaclass=BSCA):
         parameter values.
     """
 
-    #: Step engine: ``"auto"`` (the fused LDS-tile engine when the model
-    #: qualifies, else ``"plain"``), ``"fused"``, ``"plain"`` (the
-    #: reference's structure), or ``"fused_generic"``/``"fused_unfolded"``
-    #: (other forms of the fused kernel, for checking); an experiment may
-    #: override it.
+    #: Step engine: ``"auto"`` (the bit-sliced engine for Life-like
+    #: models on a 1-bit state, else the fused LDS-tile engine when the
+    #: model qualifies, else ``"plain"``), ``"bits"``, ``"fused"``,
+    #: ``"plain"`` (the reference's structure), ``"fused_or_plain"``, or
+    #: ``"fused_generic"``/``"fused_unfolded"`` (other forms of the fused
+    #: kernel, for checking); an experiment may override it.
     engine = "auto"
     #: Floating-point contraction of the generated code: ``"off"`` keeps
     #: every ``a * b + c`` as two roundings, ``"fast"`` lets the compiler

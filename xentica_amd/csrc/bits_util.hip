@@ -117,6 +117,35 @@ __global__ __launch_bounds__(kBlock) void low_bytes_kernel(const int32_t *__rest
     }
 }
 
+// Four cells per thread from a packed plane: their 0/1 bytes into `cells`
+// (one 4-byte store) and their colours into `col` (one of two int3 values
+// by state, three 16-byte stores); either output may be null.
+__global__ __launch_bounds__(kBlock) void expand_kernel(const uint32_t *__restrict__ bits,
+                                                        uint8_t *__restrict__ cells,
+                                                        int *__restrict__ col,
+                                                        long long n_quads, int4 c0a, int4 c1a) {
+    const long long q = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (q >= n_quads) return;
+    const uint32_t nib = (bits[q >> 3] >> ((q & 7) * 4)) & 0xFu;
+    if (cells)
+        *reinterpret_cast<uint32_t *>(cells + 4 * q) =
+            (nib & 1u) | ((nib & 2u) << 7) | ((nib & 4u) << 14) | ((nib & 8u) << 21);
+    if (col) {
+        int v[12];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool on = (nib >> k) & 1u;
+            v[3 * k] = on ? c1a.x : c0a.x;
+            v[3 * k + 1] = on ? c1a.y : c0a.y;
+            v[3 * k + 2] = on ? c1a.z : c0a.z;
+        }
+        int4 *dst = reinterpret_cast<int4 *>(col + 12 * q);
+        dst[0] = make_int4(v[0], v[1], v[2], v[3]);
+        dst[1] = make_int4(v[4], v[5], v[6], v[7]);
+        dst[2] = make_int4(v[8], v[9], v[10], v[11]);
+    }
+}
+
 unsigned groups_for(long long items) {
     return static_cast<unsigned>((items + kBlock - 1) / kBlock);
 }
@@ -141,6 +170,24 @@ int xen_bits_unpack(const uint32_t *bits, uint8_t *cells, long long n_cells, xen
     hipLaunchKernelGGL(unpack_kernel, dim3(groups_for(words)), dim3(kBlock), 0,
                        reinterpret_cast<hipStream_t>(stream), bits, cells, words);
     return xen::check(hipGetLastError(), "unpack_kernel launch");
+}
+
+int xen_bits_expand(const uint32_t *bits, uint8_t *cells, int32_t *colors, long long n_cells,
+                    const int32_t *color_dead, const int32_t *color_alive, xen_stream stream) {
+    if (n_cells % 32) return xen::fail("xen_bits_expand: n_cells must be a multiple of 32", 22);
+    if (colors && (!color_dead || !color_alive))
+        return xen::fail("xen_bits_expand: colours need both colour values", 22);
+    if ((reinterpret_cast<uintptr_t>(cells) & 3) || (reinterpret_cast<uintptr_t>(colors) & 15))
+        return xen::fail("xen_bits_expand: cells must be 4-byte and colors 16-byte aligned", 22);
+    const long long quads = n_cells / 4;
+    if (!quads || (!cells && !colors)) return 0;
+    const int4 c0 = colors ? make_int4(color_dead[0], color_dead[1], color_dead[2], 0)
+                           : make_int4(0, 0, 0, 0);
+    const int4 c1 = colors ? make_int4(color_alive[0], color_alive[1], color_alive[2], 0)
+                           : make_int4(0, 0, 0, 0);
+    hipLaunchKernelGGL(expand_kernel, dim3(groups_for(quads)), dim3(kBlock), 0,
+                       reinterpret_cast<hipStream_t>(stream), bits, cells, colors, quads, c0, c1);
+    return xen::check(hipGetLastError(), "expand_kernel launch");
 }
 
 int xen_low_bytes(const int32_t *src, uint8_t *dst, long long n, xen_stream stream) {

@@ -1,0 +1,208 @@
+"""
+The ``bits`` engine: Life-like models stepped on the bit-sliced kernels.
+
+A model qualifies when :class:`runtime.hostc.LifeLikeProbe` shows, from
+its own generated code and for the parameter values in force, that its
+step is ``next = born(count) | sustain(count) & state`` on a 1-bit state
+with a colour that is a function of the new state (the reference's
+``GameOfLife`` and ``LifelikeCA``, ``examples/game_of_life.py:42-70,
+110-131``, and any model written the same way).  The state is then held
+packed, 32 cells per 32-bit word, and advanced by ``xen_life_bits_run``
+(2-D, ``depth`` steps per launch in registers) or ``xen_life3d_bits_step``
+(3-D, the reference's 26-offset Moore list); ``step()`` defers steps
+until a launch is full, a parameter changes or the state is read.
+
+The reference's arrays stay the model's interface: reading
+``gpu.arrays.cells`` or ``gpu.arrays.colors`` (``.get()``/``.set()``),
+``render()``, ``save()`` and batches first expand the packed state into
+the main plane and the colours in one pass (``xen_bits_expand``), and the
+next step packs the main plane again.  Steps whose parameter values make
+the model something other than a Life-like rule, or that start from
+stored colours the colour proof does not cover, run on the fused engine
+(or the plain one) instead, from the same arrays.
+"""
+import numpy as np
+
+from xentica_amd.core.exceptions import XenticaException
+from xentica_amd.runtime import hip, kernels
+from xentica_amd.runtime.hostc import LifeLikeProbe
+
+__all__ = ['BitsEngine', 'eligible_size']
+
+
+def eligible_size(size):
+    """Whether the bit-sliced kernels take a field of ``size``."""
+    if len(size) == 2:
+        width, height = size
+        return width % 32 == 0 and width > 0 and height > 0 and \
+            width * height // 8 < 1 << 31
+    if len(size) == 3:
+        width, height, depth = size
+        return (width % 32 == 0 and width > 0 and 64 % (width // 32) == 0
+                and height > 0 and depth > 0
+                and width * height * depth // 8 < 1 << 31)
+    return False
+
+
+class BitsEngine:
+    """Single-model engine on the bit-sliced Life-like kernels."""
+
+    name = "bits"
+
+    def __init__(self, model, depth=8, strips=0):
+        self.model = model
+        if not eligible_size(model.size):
+            raise XenticaException(
+                "bits engine: the field %s does not fit the bit-sliced "
+                "kernels (width a multiple of 32; in 3-D width / 32 "
+                "dividing 64)" % (model.size, ))
+        self.probe = LifeLikeProbe(model)
+        if self.probe.rule(self.probe.values()) is None:
+            raise XenticaException(
+                "bits engine: the model is not a Life-like rule for its "
+                "parameter values")
+        #: steps per launch (2-D) and row strips (0: the library's choice)
+        self.depth, self.strips = int(depth), int(strips)
+        self.dims = len(model.size)
+        self.n = model.cells_num
+        self._bufs = None
+        self._cur = 0
+        self._packed = False        # the packed buffer holds the state
+        self._stale = False         # cells and colours lag the packed state
+        self._pending = 0
+        self._pending_rule = None
+        self._fallback = None
+        self._in_fallback = False
+        self._hook()
+
+    def _hook(self):
+        arrays = self.model.gpu.arrays
+        arrays.cells.before_read = self.flush
+        arrays.colors.before_read = self.flush
+
+    # -- the engine interface -------------------------------------------
+
+    def launch_elementwise(self, kernel, args, count):
+        """Grid-stride launch (render, through the plain kernels)."""
+        from xentica_amd.runtime.engines import elementwise_grid, BLOCK
+        kernel(*args, grid=(elementwise_grid(count), 1, 1),
+               block=(BLOCK, 1, 1))
+
+    def step(self):
+        """Queue one step with the current parameter values."""
+        rule = self.probe.rule(self.probe.values())
+        if rule is None or (not self._packed and not self._colors_admitted()):
+            self._fallback_step()
+            return
+        if self._in_fallback:
+            self._fallback.flush()
+            self._in_fallback = False
+        if self._pending and rule != self._pending_rule:
+            self._launch()
+        self._pending_rule = rule
+        self._pending += 1
+        if self.dims == 3 or self._pending >= self.depth:
+            self._launch()
+
+    def finish(self):
+        """Launch the queued steps without expanding the state (the
+        arrays are brought up to date by the next read)."""
+        if self._pending:
+            self._launch()
+        if self._in_fallback:
+            self._fallback.flush()
+
+    def flush(self):
+        """Run the queued steps; leave the state in the reference's arrays."""
+        self.finish()
+        if self._stale:
+            self._expand()
+
+    def load_state(self, cells, colors):
+        """Replace the state (queued steps are applied first)."""
+        self.flush()
+        arrays = self.model.gpu.arrays
+        arrays.cells.set(cells)
+        arrays.colors.set(colors)
+        self._packed = False
+
+    # -- internals --------------------------------------------------------
+
+    def _colors_admitted(self):
+        plan = self.probe.plan
+        if plan == ():
+            return True
+        bounds = self.model.gpu.arrays.colors.bounds
+        return bounds is not None and all(
+            low <= lo and hi <= high
+            for (lo, hi), (low, high) in zip(bounds, plan))
+
+    def _fallback_step(self):
+        """One step on the fused (or plain) engine from the arrays."""
+        self.flush()
+        self._packed = False
+        if self._fallback is None:
+            from xentica_amd.runtime import engines
+            self._fallback = engines.create(self.model, "fused_or_plain")
+            self._hook()
+        self._in_fallback = True
+        self._fallback.step()
+
+    def _buffers(self):
+        if self._bufs is None:
+            nbytes = self.n // 8
+            self._bufs = [hip.Buffer(nbytes), hip.Buffer(nbytes)]
+        return self._bufs
+
+    def _pack(self):
+        bufs = self._buffers()
+        plane = self.model.gpu.arrays.cells.device_ptr()
+        kernels.bits_pack(plane, bufs[self._cur].ptr, self.n)
+        self._packed = True
+
+    def _launch(self):
+        steps, (born, sustain, _) = self._pending, self._pending_rule
+        self._pending = 0
+        if not self._packed:
+            self._pack()
+        bufs = self._buffers()
+        if self.dims == 2:
+            width, height = self.model.size
+            in_b = kernels.life_bits_run(
+                bufs[self._cur].ptr, bufs[1 - self._cur].ptr, width // 32,
+                height, born, sustain, steps, self.depth, self.strips)
+            if in_b:
+                self._cur = 1 - self._cur
+        else:
+            for _ in range(steps):
+                kernels.life3d_bits_step(
+                    bufs[self._cur].ptr, bufs[1 - self._cur].ptr,
+                    self.model.size, born, sustain)
+                self._cur = 1 - self._cur
+        self._stale = True
+
+    def _expand(self):
+        """The packed state into the main plane and the colours."""
+        arrays = self.model.gpu.arrays
+        _, _, (dead, alive) = self._pending_rule
+        kernels.bits_expand(self._bufs[self._cur].ptr,
+                            arrays.cells.device_ptr(),
+                            arrays.colors.device_ptr(), self.n, dead, alive)
+        arrays.colors.bounds = tuple(
+            (min(a, b), max(a, b)) for a, b in zip(dead, alive))
+        self._stale = False
+        # the arrays may be written next (``set()``): pack them again
+        # before the next step
+        self._packed = False
+
+    @property
+    def state_ptr(self):
+        """Device address of the packed state (launch queued steps first)."""
+        self.finish()
+        if not self._packed:
+            self._pack()
+        return self._bufs[self._cur].ptr
+
+    def crc32(self):
+        """``binascii.crc32`` of the main plane, from the packed state."""
+        return kernels.crc32_bits(self.state_ptr, self.n)

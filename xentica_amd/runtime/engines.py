@@ -15,7 +15,11 @@ Step engines: how a model's step is executed on the GPU.
   one generic loop over the valid rectangle (the form used for tiles whose
   rows do not split evenly over the block; kept selectable to check the
   split form against it).
-* ``auto`` -- ``fused`` when the model qualifies, else ``plain``.
+* ``bits`` -- the bit-sliced Life-like kernels (``runtime/bits_engine.py``)
+  for models whose generated code is, exactly, a Life-like rule on a
+  1-bit state on a 2-D or 3-D Moore torus.
+* ``auto`` -- ``bits`` when the model qualifies, else ``fused`` when it
+  qualifies, else ``plain``.
 
 Engines share one interface: ``step()`` (advance one step, possibly
 deferred), ``flush()`` (finish deferred steps), ``load_state(cells,
@@ -114,7 +118,12 @@ def _fused_generic(model):
     return FusedEngine(model, split=False)
 
 
-def _auto(model):
+def _bits(model):
+    from xentica_amd.runtime.bits_engine import BitsEngine
+    return BitsEngine(model)
+
+
+def _fused_or_plain(model):
     """The fused engine when the model qualifies, else the plain one."""
     from xentica_amd.runtime.fused import FusedEngine, eligible
     ok, _ = eligible(model)
@@ -126,11 +135,24 @@ def _auto(model):
     return PlainEngine(model)
 
 
+def _auto(model):
+    """The bit-sliced engine, else the fused one, else the plain one."""
+    from xentica_amd.runtime.bits_engine import BitsEngine, eligible_size
+    if eligible_size(model.size):
+        try:
+            return BitsEngine(model)
+        except XenticaException:
+            pass
+    return _fused_or_plain(model)
+
+
 ENGINES = {
     'plain': PlainEngine,
     'fused': _fused,
     'fused_unfolded': _fused_unfolded,
     'fused_generic': _fused_generic,
+    'bits': _bits,
+    'fused_or_plain': _fused_or_plain,
     'auto': _auto,
 }
 

@@ -14,7 +14,7 @@ from xentica_amd.runtime.crc import combine_partials
 __all__ = [
     'life_bits_launch', 'life_bits_run', 'life3d_bits_step',
     'life3d_u8_step', 'life3d_u8_direct_step', 'bits_pack', 'bits_unpack',
-    'crc32_u8', 'crc32_bits', 'CRC_CHUNK', 'low_bytes',
+    'crc32_u8', 'crc32_bits', 'CRC_CHUNK', 'low_bytes', 'bits_expand',
 ]
 
 _vp = ctypes.c_void_p
@@ -32,6 +32,7 @@ _SIGNATURES = {
     'xen_bits_pack': [_vp, _vp, _ll, _vp],
     'xen_bits_unpack': [_vp, _vp, _ll, _vp],
     'xen_low_bytes': [_vp, _vp, _ll, _vp],
+    'xen_bits_expand': [_vp, _vp, _vp, _ll, _vp, _vp, _vp],
     'xen_crc32_u8_partials': [_vp, _ll, _ll, _vp, _vp],
     'xen_crc32_bits_partials': [_vp, _ll, _ll, _vp, _vp],
 }
@@ -106,6 +107,16 @@ def bits_unpack(bits, cells, n_cells, stream=None):
     """Packed bits -> uint8 plane, on the device."""
     hip.check(_lib().xen_bits_unpack(bits, cells, n_cells, _s(stream)),
               "xen_bits_unpack")
+
+
+def bits_expand(bits, cells, colors, n_cells, color_dead=(0, 0, 0),
+                color_alive=(0, 0, 0), stream=None):
+    """Packed bits -> uint8 plane and/or int3 colours by state (device
+    pointers; ``cells`` or ``colors`` may be None)."""
+    dead = (ctypes.c_int32 * 3)(*color_dead)
+    alive = (ctypes.c_int32 * 3)(*color_alive)
+    hip.check(_lib().xen_bits_expand(bits, cells, colors, n_cells, dead,
+                                     alive, _s(stream)), "xen_bits_expand")
 
 
 def _partials_buffer(n_chunks):
