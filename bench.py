@@ -276,32 +276,48 @@ def render_leg(steps=8, frames=5, launches=20):
     }
 
 
-def batched(steps=1000, plain_steps=10, tiles=None, depths=None):
+def batched(steps=1000, plain_steps=10, cpu=True):
     """
     The batch workload: 256 Life-like tori, sides 64-2048, own rules.
 
-    Batched: one persistent fused launch per ``depth`` steps over every
-    instance (tile and depth chosen by the autotuner), the 1000-step run
-    captured in a HIP graph.  Baseline: the per-instance
-    plain launch loop (two launches per instance per step), timed over
-    ``plain_steps`` steps.
+    Bit-sliced batch: one launch per ``depth`` steps over every instance
+    (depth and strip height chosen by the autotuner), the run captured in
+    a HIP graph.  Checked after the run: every instance's state equals the
+    fused LDS-tile batch's (an independent kernel) after the same steps.
+    Baselines: the fused batch, and the per-instance plain launch loop (two
+    launches per instance per step) over ``plain_steps`` steps.
     """
     from examples.workloads import LifeLikeRule, batch_experiments
-    from xentica_amd.runtime import hip
-    from xentica_amd.runtime.fused import autotune
+    from xentica_amd.runtime import hip, kernels
+    from xentica_amd.runtime.bits_batch import autotune as bits_autotune
+    from xentica_amd.runtime.fused import Batch
     exps = batch_experiments()
     models = [LifeLikeRule(exp) for exp in exps]
     total = sum(model.cells_num for model in models)
-    batch, tuned = autotune(models, log=log, **_tune_space(tiles, depths))
-    batch.run(steps)                       # capture, warm up
-    batch.synchronize()
-    start, stop = hip.Event(), hip.Event()
-    start.record(batch.stream)
-    batch.run(steps)
-    stop.record(batch.stream)
-    stop.synchronize()
-    ms = start.elapsed_ms(stop)
+    batch, tuned = bits_autotune(models, log=log)
+
+    def timed_run(item):
+        item.run(steps)                       # capture, warm up
+        item.synchronize()
+        start, stop = hip.Event(), hip.Event()
+        start.record(item.stream)
+        item.run(steps)
+        stop.record(item.stream)
+        stop.synchronize()
+        return start.elapsed_ms(stop)
+
+    ms = timed_run(batch)
     value = total * steps / (ms / 1e3)
+    batch.scatter()
+    bits_crcs = [kernels.crc32_u8(m.gpu.arrays.cells.device_ptr(),
+                                  m.cells_num) for m in models]
+    fused_models = [LifeLikeRule(exp) for exp in exps]
+    fused = Batch(fused_models, tile=(64, 32), depth=8)
+    fused_ms = timed_run(fused)
+    fused_value = total * steps / (fused_ms / 1e3)
+    fused.scatter()
+    same = bits_crcs == [kernels.crc32_u8(m.gpu.arrays.cells.device_ptr(),
+                                          m.cells_num) for m in fused_models]
     plain_value = None
     if plain_steps:
         plain_models = []
@@ -322,23 +338,62 @@ def batched(steps=1000, plain_steps=10, tiles=None, depths=None):
                 model.step()
         hip.device().synchronize()
         plain_value = total * plain_steps / (time.perf_counter() - t0)
-    return {
+    out = {
         "workload": "256 Life-like tori, sides log-uniform 64-2048, random "
                     "9-bit birth/survival masks, density 0.1-0.6, %d steps"
                     % steps,
         "cells": total, "value": value, "unit": "cell-updates/s",
-        "ms": ms, "tile": list(batch.program.tile),
-        "depth": batch.program.depth,
-        "algorithmic_bytes_per_cell_update": 2.0,
-        "algorithmic_GBps": value * 2.0 / 1e9,
-        "roofline_frac": value * 2.0 / 1e9 / HBM_PEAK_GBS,
-        "colour_on_last_step_only": bool(
-            batch.program.admits(batch.color_bounds)),
-        "autotune": {"%dx%d/d%d" % (t[0], t[1], d): v
-                     for (t, d), v in tuned.items()},
+        "ms": ms, "engine": "bit-sliced batch, depth %d, strips of %d rows, "
+                            "%d waves, graph" % (batch.depth,
+                                                 batch.strip_rows,
+                                                 batch.n_waves),
+        "lane_use": batch.lane_use(),
+        "algorithmic_bytes_per_cell_update": 0.25,
+        "algorithmic_GBps": value * 0.25 / 1e9,
+        "roofline_frac": value * 0.25 / 1e9 / HBM_PEAK_GBS,
+        "uint8_layout_GBps": value * 2.0 / 1e9,
+        "autotune": {"d%d/r%d" % key: rate for key, rate in tuned.items()},
+        "instances_equal_fused_batch": same,
+        "fused_batch": {"value": fused_value, "unit": "cell-updates/s",
+                        "ms": fused_ms, "tile": [64, 32], "depth": 8},
         "plain_loop_value": plain_value, "plain_loop_steps": plain_steps,
         "speedup_vs_plain": value / plain_value if plain_value else None,
+        "speedup_vs_fused_batch": value / fused_value,
     }
+    if cpu:
+        out["cpu_baseline"] = batch_cpu_baseline(exps)
+    return out
+
+
+def batch_cpu_baseline(exps, sample=16, steps=10):
+    """Every 16th instance by side, each model's generated code on the
+    host cores for ``steps`` steps (the rate over the sample's cells)."""
+    from examples.workloads import LifeLikeRule
+    from xentica_amd.runtime.hostc import PlainHost, host_threads
+    order = sorted(range(len(exps)), key=lambda idx: exps[idx].size[0])
+    picks = order[::max(1, len(order) // sample)][:sample]
+    cores = host_threads()
+    cells = 0
+    elapsed = 0.0
+    for idx in picks:
+        exp = exps[idx]
+        model = LifeLikeRule(type("Host%d" % idx, (exp, ),
+                                  {"engine": "plain"}))
+        model.rule = exp.rule
+        host = PlainHost(model)
+        host.load(model.gpu.arrays.cells.get(), model.gpu.arrays.colors.get())
+        host.step(1, threads=cores)
+        t0 = time.perf_counter()
+        host.step(steps, threads=cores)
+        elapsed += time.perf_counter() - t0
+        cells += model.cells_num
+    return {"value": cells * steps / elapsed, "unit": "cell-updates/s",
+            "cores": cores, "kind": "port",
+            "sample": "%d of the 256 instances (every 16th by side, %d "
+                      "cells), %d steps each after 1 warm-up step: the "
+                      "generated emit/absorb code compiled for the host "
+                      "(g++ -O3 -fopenmp), one instance after another"
+                      % (len(picks), cells, steps)}
 
 
 def life3d(steps=200, cpu=True):
@@ -692,18 +747,19 @@ def main():
     if args.only:
         log("%s only ..." % args.only)
         kwargs = ({"cpu": not args.no_cpu}
-                  if args.only in ("life512", "life3d", "energy") else {})
+                  if args.only in ("life512", "life3d", "energy", "batch")
+                  else {})
         if args.leg_steps and args.only in ("life512", "life3d", "energy",
                                             "batch"):
             kwargs["steps"] = args.leg_steps
-        if args.only in ("energy", "batch"):
+        if args.only == "energy":
             if args.fused_tile:
                 kwargs["tiles"] = (tuple(int(v) for v in
                                          args.fused_tile.split("x")), )
             if args.fused_depth:
                 kwargs["depths"] = (args.fused_depth, )
-            if args.no_plain:
-                kwargs["plain_steps"] = 0
+        if args.only in ("energy", "batch") and args.no_plain:
+            kwargs["plain_steps"] = 0
         print(json.dumps({args.only: _LEGS[args.only](**kwargs)}), flush=True)
         return 0
 
@@ -789,7 +845,7 @@ def main():
         result["energy_life"] = energy_life(cpu=not args.no_cpu)
     if not args.no_batch:
         log("batched workload ...")
-        result["batched"] = batched()
+        result["batched"] = batched(cpu=not args.no_cpu)
     if not args.no_surrogate:
         log("surrogate ...")
         result["surrogate"] = surrogate()

@@ -42,6 +42,34 @@ int xen_life_bits_run(void *buf_a, void *buf_b, int width_words, int height,
                       uint32_t born, uint32_t sustain, long long steps, int depth,
                       int strips, int *result_in_b, xen_stream stream);
 
+/* ---- bit-sliced batch of Life-like tori of any width ----
+ * Replaces one emit+absorb launch pair per instance per step of the
+ * reference's LifelikeCA (examples/game_of_life.py:183-203) for many
+ * independent 2-D Moore tori at once: instance i (width w_i >= 32, any
+ * height, its own birth/survival masks) is stored at word base_i as rows of
+ * pitch_i = ceil(w_i / 32) uint32 words (cell x of a row in bit x % 32 of
+ * word x / 32, the bits past w_i zero).  One launch advances every instance
+ * `depth` (1..8) steps from src to dst (the whole batch, total_bytes < 2
+ * GiB).  insts: two int4 per instance, (base_i, pitch_i, w_i, h_i) and
+ * (born_i, sustain_i, 0, 0).  lanes: 64 int4 per wave, one per lane:
+ * (instance or -1 for an idle lane, extended word column c -- the 32 cells
+ * from cell 32 c modulo w --, first output row, output rows or 0 for a lane
+ * that stores nothing); wave_rows[w]: the most output rows of any lane of
+ * wave w.  The lanes of a column tile of at most 62 stored words sit next
+ * to each other with one halo lane on each side (c = first - 1, last + 1). */
+int xen_life_batch_launch(const void *src, void *dst, long long total_bytes, const void *insts,
+                          const void *lanes, const int *wave_rows, int n_waves, int depth,
+                          xen_stream stream);
+/* Row-padded packing of one instance: a uint8 plane of width x height (0/1
+ * values) into rows of pitch = ceil(width / 32) words, and back into the
+ * plane and/or int3 colours by state (color_dead / color_alive: host
+ * pointers to three ints; cells or colors may be NULL). */
+int xen_bits_pack_rows(const uint8_t *cells, uint32_t *bits, int width, int height, int pitch,
+                       xen_stream stream);
+int xen_bits_expand_rows(const uint32_t *bits, uint8_t *cells, int32_t *colors, int width,
+                         int height, int pitch, const int32_t *color_dead,
+                         const int32_t *color_alive, xen_stream stream);
+
 /* ---- 3-D Life-like engines (torus, the reference's 3-D Moore list) ----
  * The reference's 3-D MooreNeighborhood keeps the centre and drops (1,1,1)
  * (xentica/core/topology/neighborhood.py:159-167); the count of a cell is

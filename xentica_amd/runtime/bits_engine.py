@@ -5,8 +5,8 @@ A model qualifies when :class:`runtime.hostc.LifeLikeProbe` shows, from
 its own generated code and for the parameter values in force, that its
 step is ``next = born(count) | sustain(count) & state`` on a 1-bit state
 with a colour that is a function of the new state (the reference's
-``GameOfLife`` and ``LifelikeCA``, ``examples/game_of_life.py:42-70,
-110-131``, and any model written the same way).  The state is then held
+``GameOfLife`` and ``LifelikeCA``, ``examples/game_of_life.py:15-90,
+183-203``, and any model written the same way).  The state is then held
 packed, 32 cells per 32-bit word, and advanced by ``xen_life_bits_run``
 (2-D, ``depth`` steps per launch in registers) or ``xen_life3d_bits_step``
 (3-D, the reference's 26-offset Moore list); ``step()`` defers steps
@@ -125,6 +125,13 @@ class BitsEngine:
         arrays.cells.set(cells)
         arrays.colors.set(colors)
         self._packed = False
+
+    def invalidate(self):
+        """The arrays were written directly (a batch's ``scatter()``): the
+        next step packs them again."""
+        self._pending = 0
+        self._packed = False
+        self._stale = False
 
     # -- internals --------------------------------------------------------
 
