@@ -80,11 +80,6 @@ int xen_bits_expand_rows(const uint32_t *bits, uint8_t *cells, int32_t *colors, 
  * advances one step from src to dst. */
 int xen_life3d_bits_step(const void *src, void *dst, int width, int height, int depth,
                          uint32_t born, uint32_t sustain, int z_tiles, xen_stream stream);
-/* Same rule on one uint8 per cell (value in bit 0; output bytes 0 or 1):
- * the bit-sliced kernel with byte loads and stores, same width limits,
- * field < 2 GiB of bytes. */
-int xen_life3d_u8_step(const void *src, void *dst, int width, int height, int depth,
-                       uint32_t born, uint32_t sustain, xen_stream stream);
 /* Same rule on one uint8 per cell, one thread per cell reading its 26
  * neighbours directly: an independent restatement for cross-checks, any
  * width. */

@@ -103,3 +103,28 @@ def test_model_seed_matches_oracle():
     assert cells.size == model.cells_num * 9 + 1
     assert np.array_equal(cells[:model.cells_num],
                           O.gol_experiment_field().cells)
+
+
+def test_primordial_soup_in_chunks_equals_one_draw(monkeypatch):
+    """A board larger than the chunk is drawn in chunks with the values of
+    one draw of the whole board (the legacy stream continues), for a plain
+    RandInt and for an expression of per-cell draws; a constant draw in
+    the expression keeps the one-draw path."""
+    import numpy as np
+    from xentica_amd import seeds
+    from examples.workloads import LifeLike3D, Life3DExperiment
+
+    class Soup(Life3DExperiment):
+        size = (64, 20, 10)
+
+    exprs = [seeds.random.RandInt(0, 1),
+             (seeds.random.RandInt(0, 9) + 7) >> 4,
+             seeds.random.RandInt(0, 1) * seeds.random.RandInt(0, 1, True)]
+    for expr in exprs:
+        Soup.seed = seeds.patterns.PrimordialSoup(vals={"state": expr})
+        whole = LifeLike3D(Soup).gpu.arrays.cells.get()
+        monkeypatch.setattr(seeds.patterns.PrimordialSoup, "CHUNK", 999)
+        Soup.seed = seeds.patterns.PrimordialSoup(vals={"state": expr})
+        chunked = LifeLike3D(Soup).gpu.arrays.cells.get()
+        monkeypatch.undo()
+        assert np.array_equal(whole, chunked)

@@ -69,8 +69,9 @@ def test_seed_chunks_equal_pattern():
 
 def test_full_size_bits_vs_u8():
     """
-    1024^3, 10 % density, 200 steps: the packed kernel, the byte-I/O kernel
-    and the direct 26-load stencil agree every 25 steps.
+    1024^3, 10 % density, 200 steps: the packed kernel, the uint8 plane
+    stepped packed (packed once, expanded at every read) and the direct
+    26-load stencil agree every 25 steps.
     """
     size = (1024, 1024, 1024)
     cells = life3d_cells(size)
@@ -106,3 +107,23 @@ def test_wide_rows_and_z_tiles_vs_oracle(size):
         for idx, field in enumerate(fields):
             field.step()
             assert field.crc32() == want[step], (idx, step + 1)
+
+
+def test_dsl_1024_cube_on_the_bits_engine():
+    """LifeLike3D at 1024^3 through ``model.step()``: the default engine is
+    the bit-sliced one (only the main plane is on the device), and its
+    state equals the packed kernel's run from the same cells every 50
+    steps for 200 steps."""
+    from xentica_amd.runtime.bits_engine import BitsEngine
+    print("LifeLike3D 1024^3 through the DSL", flush=True)
+    model = LifeLike3D(Life3DExperiment)
+    assert isinstance(model._engine, BitsEngine)
+    size = Life3DExperiment.size
+    field = Life3DField(size, B5, S45)
+    field.set_cells(life3d_cells(size))
+    for block in range(4):
+        for _ in range(50):
+            model.step()
+        field.step(50)
+        assert model._engine.crc32() == field.crc32(), 50 * (block + 1)
+    assert not model.gpu.arrays.cells._buffer  # buffer planes never allocated

@@ -27,7 +27,6 @@ from copy import deepcopy
 import numpy as np
 
 from xentica_amd.bridge import MoireBridge
-from xentica_amd.seeds.random import LocalRandom
 from xentica_amd.core import _context
 from xentica_amd.core.addressing import PlainAddressing, TileAddressing
 from xentica_amd.core.properties import Property, ContainerProperty, UINT128
@@ -438,14 +437,20 @@ class GPUKernels:
 
 
 class GPUArrays:
-    """The device arrays of a model (``base.py:544-569``)."""
+    """
+    The device arrays of a model (``base.py:544-569``): ``cells`` in the
+    reference's layout (``n * (k + 1) + 1`` words, of which the seeds write
+    the main plane and the rest starts at zero) and ``colors`` (``3 n``
+    ints, zero at the start).
+    """
 
-    def __init__(self, init_cells, init_colors):
-        """Hold the cells (reference layout) and colours."""
+    def __init__(self, init_plane, cells_total, num_cells):
+        """Hold the cells (main plane given, buffer planes zero) and the
+        colours (zero)."""
         self.img = None
-        self.colors = gpuarray.to_gpu(init_colors)
+        self.colors = gpuarray.zeros((3 * num_cells, ), dtype=np.int32)
         self.colors.track_bounds(3)
-        self.cells = gpuarray.to_gpu(init_cells)
+        self.cells = gpuarray.to_gpu(init_plane, cells_total)
 
     def init_img(self, num_cells):
         """Allocate the render image."""
@@ -455,12 +460,13 @@ class GPUArrays:
 class GPU:
     """Kernels and arrays of a model; kernels compile on first use."""
 
-    def __init__(self, source, init_cells, init_colors, fp_contract="off"):
+    def __init__(self, source, init_plane, cells_total, num_cells,
+                 fp_contract="off"):
         """Keep the source; allocate nothing on the device yet."""
         self._source = source
         self._fp_contract = fp_contract
         self._kernels = None
-        self.arrays = GPUArrays(init_cells, init_colors)
+        self.arrays = GPUArrays(init_plane, cells_total, num_cells)
 
     @property
     def kernels(self):
@@ -510,6 +516,9 @@ class CellularAutomaton(Translator, metaclass=BSCA):
     def _init_from_experiment(self, experiment_class):
         self.frame_buf = np.zeros((3, ), dtype=np.uint8)
         self.width, self.height = 0, 0
+        # imported here: the seeds import the DSL's expressions, whose
+        # package imports this module
+        from xentica_amd.seeds.random import LocalRandom
         self.random = LocalRandom(experiment_class.word)
         experiment_class.seed.random = self.random
         experiment_class.random = self.random
@@ -528,12 +537,13 @@ class CellularAutomaton(Translator, metaclass=BSCA):
         self.timestep = 0
         self.cells_num = functools.reduce(operator.mul, self.size)
         self.build_source()
-        init_colors = np.zeros((self.cells_num * 3, ), dtype=np.int32)
         cells_total = self.cells_num * (len(self.buffers) + 1) + 1
-        init_cells = np.zeros((cells_total, ), dtype=self.main.dtype)
-        experiment_class.seed.generate(init_cells, self)
-        self.gpu = GPU(self.cuda_source, init_cells, init_colors,
-                       self.fp_contract)
+        # seeds write the main plane; the buffer planes start at zero and
+        # are allocated only where an engine uses them
+        init_plane = np.zeros((self.cells_num, ), dtype=self.main.dtype)
+        experiment_class.seed.generate(init_plane, self)
+        self.gpu = GPU(self.cuda_source, init_plane, cells_total,
+                       self.cells_num, self.fp_contract)
         self.bridge = MoireBridge
         self.renderer.setup_actions(self.bridge)
         self._lock = threading.Lock()

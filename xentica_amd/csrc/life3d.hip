@@ -17,9 +17,6 @@
 //   rows, XY = 3-row y-sums (4 bits); the count of plane z is the
 //   bit-sliced sum XY(z-1) + XY(z) + XY(z+1) (5 bits) minus the (1,1,1)
 //   bit.  B5/S45 collapses to three 3-input logic ops.
-//   With kBytes the same kernel reads and writes one uint8 per cell: a
-//   lane loads its 32 cells as 32 bytes, gathers them into a word in
-//   registers, and spreads the result back to bytes on the store.
 // life3d_u8_kernel: one uint8 per cell, one thread per cell reading its 26
 //   neighbours directly -- a plain restatement kept as an independent check
 //   of the bit-sliced kernels.
@@ -174,70 +171,23 @@ struct Geometry {
     int left_src, right_src;
 };
 
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-
-// four 0/1 bytes -> 4 bits (byte k -> bit k): the masked bytes times
-// 0x01020408 put byte k's bit at bit 24 + k with no carries
-__device__ __forceinline__ uint32_t gather4(uint32_t v) {
-    return ((v & 0x01010101u) * 0x01020408u) >> 24;
-}
-
-// 4 bits -> four 0/1 bytes (bit k -> byte k): bit k of n times 0x00204081
-// lands at k + 7j, distinct positions, of which 8k is picked by the mask
-__device__ __forceinline__ uint32_t spread4(uint32_t n) {
-    return ((n & 0xFu) * 0x00204081u) & 0x01010101u;
-}
-
-// 32 cells of a row as one word: one uint32 of bits, or 32 bytes
-template <bool kBytes>
+// 32 cells of a row as one word
 __device__ __forceinline__ uint32_t load_word(__amdgpu_buffer_rsrc_t in, int voff, int soff) {
-    if constexpr (!kBytes) {
-        return __builtin_amdgcn_raw_buffer_load_b32(in, voff, soff, 0);
-    } else {
-        const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(in, voff, soff, 0);
-        const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(in, voff + 16, soff, 0);
-        return gather4(a[0]) | (gather4(a[1]) << 4) | (gather4(a[2]) << 8) |
-               (gather4(a[3]) << 12) | (gather4(b[0]) << 16) | (gather4(b[1]) << 20) |
-               (gather4(b[2]) << 24) | (gather4(b[3]) << 28);
-    }
+    return __builtin_amdgcn_raw_buffer_load_b32(in, voff, soff, 0);
 }
 
-// The byte path stores 8 bytes at a time and keeps the next instruction
-// five wait states away from each store.  Measured on MI355X: when the
-// compiler overwrote a store's data register in the very next instruction
-// (which it does where register pressure is low, e.g. the last plane of a
-// wave), the first dword of that store intermittently reached memory with
-// the new value -- 16-byte stores of the same data failed in every run of
-// 25 full-size steps, 8-byte stores in half of them, 8-byte stores with
-// these wait states in none of 100.  The scheduling barriers keep the
-// compiler from moving the overwrite across the wait.
-template <bool kBytes>
 __device__ __forceinline__ void store_word(uint32_t w, __amdgpu_buffer_rsrc_t out, int voff,
                                            int soff) {
-    if constexpr (!kBytes) {
-        __builtin_amdgcn_raw_buffer_store_b32(w, out, voff, soff, 0);
-    } else {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            u32x2 v;
-            v[0] = spread4(w >> (8 * q));
-            v[1] = spread4(w >> (8 * q + 4));
-            __builtin_amdgcn_raw_buffer_store_b64(v, out, voff + 8 * q, soff, 0);
-            __builtin_amdgcn_sched_barrier(0);
-            asm volatile("s_nop 4");
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
+    __builtin_amdgcn_raw_buffer_store_b32(w, out, voff, soff, 0);
 }
 
-template <int YS, bool kBytes>
+template <int YS>
 __device__ __forceinline__ void load_plane(const Geometry<YS> &g, int z, Plane<YS> &p) {
     uint32_t x0[YS + 2], x1[YS + 2];
     const int soff = wrap(z, g.D) * g.plane_bytes;  // wave-uniform
 #pragma unroll
     for (int r = 0; r < YS + 2; ++r) {
-        const uint32_t w = load_word<kBytes>(g.in, g.row_off[r], soff);
+        const uint32_t w = load_word(g.in, g.row_off[r], soff);
         const uint32_t wl = lane_read(w, g.left_src), wr = lane_read(w, g.right_src);
         const uint32_t l = (w << 1) | (wl >> 31);  // cells at x - 1
         const uint32_t rr = (w >> 1) | (wr << 31);  // cells at x + 1
@@ -254,7 +204,7 @@ __device__ __forceinline__ void load_plane(const Geometry<YS> &g, int z, Plane<Y
 // Output plane z from the y-sums of planes z-1 (a), z (b), z+1 (c);
 // out_off[y] is the row's per-lane offset, or the buffer size (dropped by
 // the range check) for rows beyond the field.
-template <int YS, bool kFast, bool kBytes>
+template <int YS, bool kFast>
 __device__ __forceinline__ void emit_plane(const Plane<YS> &a, const Plane<YS> &b,
                                            const Plane<YS> &c, const Rule3<kFast> &rule,
                                            __amdgpu_buffer_rsrc_t out, const int (&out_off)[YS],
@@ -263,14 +213,11 @@ __device__ __forceinline__ void emit_plane(const Plane<YS> &a, const Plane<YS> &
     for (int y = 0; y < YS; ++y) {
         const Count5 full = add3_4(a.xy[y], b.xy[y], c.xy[y]);
         const uint32_t next = rule.apply(full, c.right[y], b.raw[y]);
-        store_word<kBytes>(next, out, out_off[y], soff);
+        store_word(next, out, out_off[y], soff);
     }
 }
 
-// kBytes: the field is one uint8 per cell (value in bit 0; the output bytes
-// are 0 or 1); each lane word still covers 32 cells, loaded as 32 bytes and
-// gathered into bits in registers.
-template <int YS, bool kFast, bool kBytes>
+template <int YS, bool kFast>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void life3d_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, int Wd,
                         int H, int D, int y_tiles, int z_tiles, uint32_t born,
@@ -284,7 +231,7 @@ void life3d_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__
     const int yt = wave % y_tiles, zt = wave / y_tiles;
     const int z_lo = static_cast<int>((static_cast<long long>(zt) * D) / z_tiles);
     const int z_hi = static_cast<int>((static_cast<long long>(zt + 1) * D) / z_tiles);
-    constexpr int kUnit = kBytes ? 32 : 4;  // bytes per 32-cell lane word
+    constexpr int kUnit = 4;  // bytes per 32-cell lane word
     const unsigned bytes = static_cast<unsigned>(static_cast<size_t>(Wd) * H * D * kUnit);
     const int row_bytes = Wd * kUnit;
     const int y0 = (yt * G + group) * YS;
@@ -306,23 +253,23 @@ void life3d_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__
     const int pb = g.plane_bytes;
 
     Plane<YS> p0, p1, p2;
-    load_plane<YS, kBytes>(g, z_lo - 1, p0);
-    load_plane<YS, kBytes>(g, z_lo, p1);
+    load_plane<YS>(g, z_lo - 1, p0);
+    load_plane<YS>(g, z_lo, p1);
     int z = z_lo;  // next output plane; plane z+1 is loaded next
     for (; z + 2 < z_hi; z += 3) {
-        load_plane<YS, kBytes>(g, z + 1, p2);
-        emit_plane<YS, kFast, kBytes>(p0, p1, p2, rule, out, out_off, z * pb);
-        load_plane<YS, kBytes>(g, z + 2, p0);
-        emit_plane<YS, kFast, kBytes>(p1, p2, p0, rule, out, out_off, (z + 1) * pb);
-        load_plane<YS, kBytes>(g, z + 3, p1);
-        emit_plane<YS, kFast, kBytes>(p2, p0, p1, rule, out, out_off, (z + 2) * pb);
+        load_plane<YS>(g, z + 1, p2);
+        emit_plane<YS, kFast>(p0, p1, p2, rule, out, out_off, z * pb);
+        load_plane<YS>(g, z + 2, p0);
+        emit_plane<YS, kFast>(p1, p2, p0, rule, out, out_off, (z + 1) * pb);
+        load_plane<YS>(g, z + 3, p1);
+        emit_plane<YS, kFast>(p2, p0, p1, rule, out, out_off, (z + 2) * pb);
     }
     if (z < z_hi) {
-        load_plane<YS, kBytes>(g, z + 1, p2);
-        emit_plane<YS, kFast, kBytes>(p0, p1, p2, rule, out, out_off, z * pb);
+        load_plane<YS>(g, z + 1, p2);
+        emit_plane<YS, kFast>(p0, p1, p2, rule, out, out_off, z * pb);
         if (z + 1 < z_hi) {
-            load_plane<YS, kBytes>(g, z + 2, p0);
-            emit_plane<YS, kFast, kBytes>(p1, p2, p0, rule, out, out_off, (z + 1) * pb);
+            load_plane<YS>(g, z + 2, p0);
+            emit_plane<YS, kFast>(p1, p2, p0, rule, out, out_off, (z + 1) * pb);
         }
     }
 }
@@ -350,13 +297,13 @@ __global__ __launch_bounds__(256) void life3d_u8_kernel(const uint8_t *__restric
     }
 }
 
-template <int YS, bool kFast, bool kBytes>
+template <int YS, bool kFast>
 int launch_bits(const uint32_t *src, uint32_t *dst, int Wd, int H, int D, int z_tiles,
                 uint32_t born, uint32_t sustain, hipStream_t s) {
     const int G = kWave / Wd;
     const int y_tiles = (H + G * YS - 1) / (G * YS);
     const long long waves = static_cast<long long>(y_tiles) * z_tiles;
-    hipLaunchKernelGGL((life3d_bits_kernel<YS, kFast, kBytes>),
+    hipLaunchKernelGGL((life3d_bits_kernel<YS, kFast>),
                        dim3(static_cast<unsigned>((waves + 3) / 4)), dim3(256), 0, s, src, dst,
                        Wd, H, D, y_tiles, z_tiles, born, sustain);
     return xen::check(hipGetLastError(), "life3d_bits_kernel launch");
@@ -388,8 +335,7 @@ int auto_z_tiles(int Wd, int height, int depth) {
     return z < 1 ? 1 : static_cast<int>(z);
 }
 
-// Validates a packed or byte field and launches the bit-sliced kernel.
-template <bool kBytes>
+// Validates a packed field and launches the bit-sliced kernel.
 int step3d(const char *name, const void *src, void *dst, int width, int height, int depth,
            uint32_t born, uint32_t sustain, int z_tiles, xen_stream stream) {
     const std::string who(name);
@@ -398,7 +344,7 @@ int step3d(const char *name, const void *src, void *dst, int width, int height, 
     if (Wd < 1 || Wd > 64 || (64 % Wd))
         return xen::fail(who + ": width / 32 must divide 64", 22);
     if (height < 1 || depth < 1) return xen::fail(who + ": empty field", 22);
-    const long long unit = kBytes ? 32 : 4;
+    const long long unit = 4;
     if (static_cast<long long>(Wd) * height * depth * unit >= (1LL << 31))
         return xen::fail(who + ": field must be smaller than 2 GiB", 22);
     if (src == dst) return xen::fail(who + ": src and dst must differ", 22);
@@ -410,9 +356,9 @@ int step3d(const char *name, const void *src, void *dst, int width, int height, 
     const bool fast = (born & 0x7FFFFFFu) == (1u << 5) &&
                       (sustain & 0x7FFFFFFu) == ((1u << 4) | (1u << 5));
     if (fast)
-        return launch_bits<8, true, kBytes>(in, out, Wd, height, depth, z_tiles, born, sustain,
+        return launch_bits<8, true>(in, out, Wd, height, depth, z_tiles, born, sustain,
                                             s);
-    return launch_bits<8, false, kBytes>(in, out, Wd, height, depth, z_tiles, born, sustain, s);
+    return launch_bits<8, false>(in, out, Wd, height, depth, z_tiles, born, sustain, s);
 }
 
 }  // namespace
@@ -421,14 +367,8 @@ extern "C" {
 
 int xen_life3d_bits_step(const void *src, void *dst, int width, int height, int depth,
                          uint32_t born, uint32_t sustain, int z_tiles, xen_stream stream) {
-    return step3d<false>("xen_life3d_bits_step", src, dst, width, height, depth, born, sustain,
+    return step3d("xen_life3d_bits_step", src, dst, width, height, depth, born, sustain,
                          z_tiles, stream);
-}
-
-int xen_life3d_u8_step(const void *src, void *dst, int width, int height, int depth,
-                       uint32_t born, uint32_t sustain, xen_stream stream) {
-    return step3d<true>("xen_life3d_u8_step", src, dst, width, height, depth, born, sustain, 0,
-                        stream);
 }
 
 int xen_life3d_u8_direct_step(const void *src, void *dst, int width, int height, int depth,

@@ -243,7 +243,7 @@ class BitsBatch:
                     "colour proof does not cover" % idx)
             width, height = self.sizes[idx]
             hip.check(lib.xen_bits_pack_rows(
-                model.gpu.arrays.cells.device_ptr(),
+                model.gpu.arrays.cells.plane_ptr(model.cells_num),
                 self.state[self.cur].ptr + 4 * base, width, height,
                 pitch_of(width), _stream(self.stream)), "xen_bits_pack_rows")
         self.stream.synchronize()
@@ -258,7 +258,8 @@ class BitsBatch:
             arrays = model.gpu.arrays
             hip.check(lib.xen_bits_expand_rows(
                 self.state[self.cur].ptr + 4 * base,
-                arrays.cells.device_ptr(), arrays.colors.device_ptr(),
+                arrays.cells.plane_ptr(model.cells_num),
+                arrays.colors.device_ptr(),
                 width, height, pitch_of(width),
                 (ctypes.c_int32 * 3)(*dead), (ctypes.c_int32 * 3)(*alive),
                 _stream(self.stream)), "xen_bits_expand_rows")

@@ -163,7 +163,7 @@ class BitsEngine:
 
     def _pack(self):
         bufs = self._buffers()
-        plane = self.model.gpu.arrays.cells.device_ptr()
+        plane = self.model.gpu.arrays.cells.plane_ptr(self.n)
         kernels.bits_pack(plane, bufs[self._cur].ptr, self.n)
         self._packed = True
 
@@ -193,7 +193,7 @@ class BitsEngine:
         arrays = self.model.gpu.arrays
         _, _, (dead, alive) = self._pending_rule
         kernels.bits_expand(self._bufs[self._cur].ptr,
-                            arrays.cells.device_ptr(),
+                            arrays.cells.plane_ptr(self.n),
                             arrays.colors.device_ptr(), self.n, dead, alive)
         arrays.colors.bounds = tuple(
             (min(a, b), max(a, b)) for a, b in zip(dead, alive))

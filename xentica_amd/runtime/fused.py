@@ -941,7 +941,7 @@ class FusedEngine:
     def _run(self, steps, values):
         self._inst.refresh(values)
         arrays = self.model.gpu.arrays
-        plane0 = arrays.cells.device_ptr()
+        plane0 = arrays.cells.plane_ptr(self.model.cells_num)
         colors = arrays.colors.device_ptr()
         src, dst = ((self._other.ptr, plane0) if self._in_other
                     else (plane0, self._other.ptr))
@@ -974,7 +974,8 @@ class FusedEngine:
         if self._in_other:
             nbytes = self.model.cells_num * self._inst.itemsize
             hip.check(hip.library().xen_memcpy_d2d(
-                self.model.gpu.arrays.cells.device_ptr(), self._other.ptr,
+                self.model.gpu.arrays.cells.plane_ptr(self.model.cells_num),
+                self._other.ptr,
                 nbytes, None))
             self._in_other = False
 
@@ -1036,7 +1037,7 @@ class Batch:
             arrays = model.gpu.arrays
             check(lib.xen_memcpy_d2d(
                 self.state[self.cur].ptr + off * self.inst.itemsize,
-                arrays.cells.device_ptr(),
+                arrays.cells.plane_ptr(model.cells_num),
                 model.cells_num * self.inst.itemsize, self.stream.handle))
             check(lib.xen_memcpy_d2d(
                 self.colors.ptr + coff * 12, arrays.colors.device_ptr(),
@@ -1053,13 +1054,15 @@ class Batch:
                                     self.inst.col_offsets):
             arrays = model.gpu.arrays
             check(lib.xen_memcpy_d2d(
-                arrays.cells.device_ptr(),
+                arrays.cells.plane_ptr(model.cells_num),
                 self.state[self.cur].ptr + off * self.inst.itemsize,
                 model.cells_num * self.inst.itemsize, self.stream.handle))
             check(lib.xen_memcpy_d2d(
                 arrays.colors.device_ptr(), self.colors.ptr + coff * 12,
                 model.cells_num * 12, self.stream.handle))
             arrays.colors.bounds = self.color_bounds
+            if hasattr(model._engine, "invalidate"):
+                model._engine.invalidate()
         self.stream.synchronize()
 
     def _launch(self, steps, fast):

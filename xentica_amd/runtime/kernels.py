@@ -13,7 +13,7 @@ from xentica_amd.runtime.crc import combine_partials
 
 __all__ = [
     'life_bits_launch', 'life_bits_run', 'life3d_bits_step',
-    'life3d_u8_step', 'life3d_u8_direct_step', 'bits_pack', 'bits_unpack',
+    'life3d_u8_direct_step', 'bits_pack', 'bits_unpack',
     'crc32_u8', 'crc32_bits', 'CRC_CHUNK', 'low_bytes', 'bits_expand',
 ]
 
@@ -27,7 +27,6 @@ _SIGNATURES = {
     'xen_life_bits_run': [_vp, _vp, _i, _i, _u32, _u32, _ll, _i, _i,
                           ctypes.POINTER(_i), _vp],
     'xen_life3d_bits_step': [_vp, _vp, _i, _i, _i, _u32, _u32, _i, _vp],
-    'xen_life3d_u8_step': [_vp, _vp, _i, _i, _i, _u32, _u32, _vp],
     'xen_life3d_u8_direct_step': [_vp, _vp, _i, _i, _i, _u32, _u32, _vp],
     'xen_bits_pack': [_vp, _vp, _ll, _vp],
     'xen_bits_unpack': [_vp, _vp, _ll, _vp],
@@ -81,13 +80,6 @@ def life3d_bits_step(src, dst, size, born, sustain, z_tiles=0, stream=None):
     hip.check(_lib().xen_life3d_bits_step(
         src, dst, size[0], size[1], size[2], born, sustain, z_tiles,
         _s(stream)), "xen_life3d_bits_step")
-
-
-def life3d_u8_step(src, dst, size, born, sustain, stream=None):
-    """One step of a uint8 3-D Life-like torus (bit-sliced, byte I/O)."""
-    hip.check(_lib().xen_life3d_u8_step(
-        src, dst, size[0], size[1], size[2], born, sustain, _s(stream)),
-        "xen_life3d_u8_step")
 
 
 def life3d_u8_direct_step(src, dst, size, born, sustain, stream=None):

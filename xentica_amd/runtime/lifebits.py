@@ -190,30 +190,46 @@ class Life3DField:
 
 class Life3DBytes:
     """
-    The same 3-D rule on one uint8 per cell.
+    The same 3-D rule on one uint8 per cell (the reference's layout of a
+    1-bit state, ``xentica/core/base.py:659-661``).
 
-    ``direct=False``: the bit-sliced kernel with byte loads and stores
-    (widths as for :class:`Life3DField`); ``direct=True``: one thread per
-    cell reading its 26 neighbours, any width (an independent check).
+    ``direct=False``: the plane is packed once before the first step, the
+    steps run on the bit-sliced kernel (widths as for
+    :class:`Life3DField`), and the plane is expanded again when it is
+    read (``state_ptr``, ``get_cells``, ``crc32``): one uint8 read and one
+    write per cell for any number of steps.  ``direct=True``: one thread
+    per cell reading its 26 neighbours, any width (an independent check).
     """
 
     def __init__(self, size, born, sustain, stream=None, direct=False):
-        self._step = (kernels.life3d_u8_direct_step if direct
-                      else kernels.life3d_u8_step)
+        self.direct = bool(direct)
         self.size = tuple(int(v) for v in size)
         self.n_cells = self.size[0] * self.size[1] * self.size[2]
         self.born, self.sustain = int(born), int(sustain)
         self.stream = stream
         self._bufs = [hip.Buffer(self.n_cells), hip.Buffer(self.n_cells)]
         self._cur = 0
+        self._packed = None if self.direct else Life3DField(
+            self.size, born, sustain, stream=stream)
+        self._in_bits = False
+
+    def _plane(self):
+        """The uint8 plane, expanded from the packed state if it is behind."""
+        if self._in_bits:
+            kernels.bits_unpack(self._packed.state_ptr,
+                                self._bufs[self._cur].ptr, self.n_cells,
+                                self.stream)
+            self._in_bits = False
+        return self._bufs[self._cur].ptr
 
     @property
     def state_ptr(self):
-        """Device address of the current state."""
-        return self._bufs[self._cur].ptr
+        """Device address of the current uint8 plane."""
+        return self._plane()
 
     def set_cells(self, cells):
         """Load a uint8 plane."""
+        self._in_bits = False
         self._bufs[self._cur].upload(
             np.ascontiguousarray(cells, dtype=np.uint8).reshape(-1) & 1,
             self.stream)
@@ -221,19 +237,31 @@ class Life3DBytes:
     def get_cells(self):
         """The state as a uint8 plane."""
         out = np.empty(self.n_cells, dtype=np.uint8)
+        self._plane()
         self._bufs[self._cur].download(out, self.stream)
         return out
 
     def crc32(self):
         """``binascii.crc32`` of the plane, on the device."""
-        return kernels.crc32_u8(self.state_ptr, self.n_cells, self.stream)
+        return kernels.crc32_u8(self._plane(), self.n_cells, self.stream)
 
     def step(self, steps=1):
         """Advance ``steps`` steps (asynchronous)."""
-        for _ in range(steps):
-            self._step(self.state_ptr, self._bufs[1 - self._cur].ptr,
-                       self.size, self.born, self.sustain, self.stream)
-            self._cur = 1 - self._cur
+        if steps <= 0:
+            return
+        if self.direct:
+            for _ in range(steps):
+                kernels.life3d_u8_direct_step(
+                    self.state_ptr, self._bufs[1 - self._cur].ptr, self.size,
+                    self.born, self.sustain, self.stream)
+                self._cur = 1 - self._cur
+            return
+        if not self._in_bits:
+            kernels.bits_pack(self._bufs[self._cur].ptr,
+                              self._packed.state_ptr, self.n_cells,
+                              self.stream)
+            self._in_bits = True
+        self._packed.step(steps)
 
     def synchronize(self):
         """Wait for queued work."""

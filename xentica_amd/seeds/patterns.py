@@ -162,8 +162,33 @@ class BigBang(RandomPattern):
         cells[np.where(area)] = bsca.pack_state(self._draw_state())
 
 
+def _per_cell_draws(value):
+    """Whether ``value`` is a number or an expression whose random leaves
+    all draw one value per cell from the NumPy stream (no ``constant``
+    draws from the stdlib stream)."""
+    from .random import RandInt
+    from xentica_amd.core.expressions import PatternExpression
+    if isinstance(value, RandInt):
+        return not value._constant  # pylint: disable=protected-access
+    if isinstance(value, PatternExpression):
+        return all(_per_cell_draws(arg) for arg in (value.arg1, value.arg2)
+                   if arg is not None)
+    return isinstance(value, (int, float, np.integer, np.floating))
+
+
 class PrimordialSoup(RandomPattern):
-    """Random values over the whole board."""
+    """
+    Random values over the whole board.
+
+    A board of more than ``CHUNK`` cells with a single field whose values
+    are all drawn per cell is filled ``CHUNK`` cells at a time: the legacy
+    ``RandomState`` stream continues across calls, so the values are the
+    ones one draw of the whole board gives, without its int64 temporaries
+    (8 bytes per cell for every intermediate value).
+    """
+
+    #: cells drawn at a time on large boards
+    CHUNK = 1 << 24
 
     def __init__(self, vals):
         """Initialize class."""
@@ -172,5 +197,16 @@ class PrimordialSoup(RandomPattern):
 
     def generate(self, cells, bsca):
         """Fill every cell."""
+        count = bsca.cells_num
+        keys = list(self.vals.keys())
+        if count > self.CHUNK and len(keys) == 1 and \
+                _per_cell_draws(self.vals._d[keys[0]]):  # pylint: disable=protected-access
+            for start in range(0, count, self.CHUNK):
+                part = min(self.CHUNK, count - start)
+                self.size = (part, )
+                cells[start:start + part] = bsca.pack_state(
+                    self._draw_state())
+            self.size = bsca.size
+            return
         self.size = bsca.size
-        cells[:bsca.cells_num] = bsca.pack_state(self._draw_state())
+        cells[:count] = bsca.pack_state(self._draw_state())

@@ -39,7 +39,8 @@ def encode_model(model, x, block_mean=None):
     """Encode the model's current main plane into ``x`` ``[1, h, w, 16]``."""
     model.flush()
     h, w = coarse_shape(model)
-    encode(model.gpu.arrays.cells.device_ptr(), 1, h, w, x, block_mean,
+    encode(model.gpu.arrays.cells.plane_ptr(model.cells_num), 1, h, w, x,
+           block_mean,
            value_mask(model))
 
 
