@@ -158,10 +158,10 @@ def life512(steps=1000, cpu=True):
     """
     Life B3/S23 512x512 (word 42, 50 %) through the drop-in API, where the
     per-step host work of ``model.step()`` dominates: ``steps`` steps on
-    the default engine (fused, launches deferred until a launch is full),
+    the default engine (bits: launches deferred until a launch is full),
     the same with the reference's test idiom (``crc32`` of
-    ``cells.get()`` after every step), and the plain engine; the hashes
-    are checked against the oracle's, which is also timed on the host.
+    ``cells.get()`` after every step), the fused and the plain engine; the
+    every-step hashes are checked against the oracle's.
     """
     import binascii
     from examples.game_of_life import GameOfLife
@@ -189,41 +189,48 @@ def life512(steps=1000, cpu=True):
         hip.device().synchronize()
         return time.perf_counter() - t0, hashes
 
+    def rate(seconds):
+        return {"value": n * steps / seconds, "unit": "cell-updates/s",
+                "us_per_step": seconds / steps * 1e6}
+
+    bits_s, _ = timed_run("bits", False)
+    hashed_s, hashes = timed_run("bits", True)
     fused_s, _ = timed_run("fused", False)
-    hashed_s, hashes = timed_run("fused", True)
     plain_s, _ = timed_run("plain", False)
     out = {
         "workload": "Life B3/S23 512x512 torus, word 42, RandInt(0, 1), "
                     "%d steps through model.step() after one warm-up step"
                     % steps,
-        "fused": {"value": n * steps / fused_s, "unit": "cell-updates/s",
-                  "us_per_step": fused_s / steps * 1e6},
-        "fused_crc_every_step": {"value": n * steps / hashed_s,
-                                 "unit": "cell-updates/s",
-                                 "us_per_step": hashed_s / steps * 1e6},
-        "plain": {"value": n * steps / plain_s, "unit": "cell-updates/s",
-                  "us_per_step": plain_s / steps * 1e6},
+        "bits": rate(bits_s), "bits_crc_every_step": rate(hashed_s),
+        "fused": rate(fused_s), "plain": rate(plain_s),
     }
+    out["crc_step_%d" % (steps + 1)] = hashes[-1]
+    if cpu:
+        # the checker of the every-step hashes runs with the CPU baseline
+        out["hashes_match_oracle"] = life512_oracle_hashes(steps) == hashes
+        cpu_exp = type("Life512Host", (Life512Experiment, ),
+                       {"engine": "plain"})
+        out["cpu_baseline"] = host_baseline(
+            GameOfLife(cpu_exp), steps,
+            "the whole workload: %d steps after 1 warm-up step, the DSL's "
+            "generated emit/absorb (MovingAverage colours) compiled for the "
+            "host, g++ -O3 -fopenmp" % steps)
+    return out
+
+
+def life512_oracle_hashes(steps):
+    """The oracle's CRC of steps 2 .. steps + 1 of Life 512^2 (the checker
+    of the timed runs, which start after one warm-up step)."""
+    from examples.game_of_life import GameOfLife
+    from examples.workloads import Life512Experiment
     from oracle import oracle
+    n = 512 * 512
     field = oracle.Field((512, 512), 8, np.uint8,
                          GameOfLife(Life512Experiment).gpu.arrays.cells.get()
                          [:n])
-    cores = len(os.sched_getaffinity(0))
-    t0 = time.perf_counter()
     want = oracle.lifelike(field, oracle.moore_deltas(2), oracle.moore_rev(2),
-                           8, 12, steps + 1, threads=cores, hashes=True)
-    cpu_s = time.perf_counter() - t0
-    # the timed runs cover steps 2 .. steps + 1 (step 1 is the warm-up)
-    out["hashes_match_oracle"] = [int(h) for h in want[1:]] == hashes
-    out["crc_step_%d" % (steps + 1)] = hashes[-1]
-    if cpu:
-        out["cpu_baseline"] = {
-            "value": n * (steps + 1) / cpu_s, "unit": "cell-updates/s",
-            "cores": cores, "kind": "port",
-            "sample": "the whole workload: %d steps with colours and a CRC "
-                      "of every step, reference plain structure, C + OpenMP"
-                      % (steps + 1)}
-    return out
+                           8, 12, steps + 1, hashes=True)
+    return [int(h) for h in want[1:]]
 
 
 def render_leg(steps=8, frames=5, launches=20):
@@ -473,45 +480,34 @@ def life3d_dsl(steps, crc_want):
             "state_equals_packed_kernel": engine.crc32() == crc_want}
 
 
-def life3d_cpu_baseline(born, sustain, side=256, steps=40):
-    """
-    The oracle on a 256^3 sample of the same workload (the reference's
-    plain layout of 1024^3 would need 27 GiB of buffer planes): same rule,
-    26-offset list and 10 % seeding, ``steps`` steps after one warm-up.
-    """
-    from examples.workloads import life3d_cells
-    from oracle import oracle
-    cores = len(os.sched_getaffinity(0))
-    size = (side, side, side)
-    field = oracle.Field(size, 26, np.uint8, life3d_cells(size))
-    args = (oracle.moore_deltas(3), oracle.moore_rev(3), born, sustain)
-    oracle.lifelike(field, *args, 1, colors=False, threads=cores)
-    t0 = time.perf_counter()
-    oracle.lifelike(field, *args, steps, colors=False, threads=cores)
-    elapsed = time.perf_counter() - t0
-    return {"value": side ** 3 * steps / elapsed, "unit": "cell-updates/s",
-            "cores": cores, "kind": "port",
-            "sample": "%d^3 torus (the 1024^3 field in the reference's plain "
-                      "layout needs 27 GiB), B5/S45 with the 26-offset list, "
-                      "10 %% density, %d steps after 1 warm-up step, 26 buffer "
-                      "planes, no colours, C + OpenMP" % (side, steps)}
+def life3d_cpu_baseline(born, sustain, side=256, steps=10):
+    """``LifeLike3D``'s generated code on the host cores on a 256^3 sample
+    of the same workload (the reference's plain layout of 1024^3 needs 27
+    GiB of buffer planes): same rule, 26-offset list and 10 % seeding."""
+    from examples.workloads import LifeLike3D, Life3DExperiment
+    cpu_exp = type("Life3DHost", (Life3DExperiment, ),
+                   {"size": (side, side, side), "engine": "plain",
+                    "born": born, "sustain": sustain})
+    return host_baseline(
+        LifeLike3D(cpu_exp), steps,
+        "%d^3 torus (the 1024^3 field in the reference's plain layout needs "
+        "27 GiB), B5/S45 with the 26-offset list, 10 %% density, %d steps "
+        "after 1 warm-up step: the DSL's generated emit/absorb (26 buffer "
+        "planes, colours) compiled for the host, g++ -O3 -fopenmp"
+        % (side, steps))
 
 
-def energy_cpu_baseline(steps=50):
-    """The oracle on the same 4096^2 EnergyLife field, ``steps`` steps
-    (colours included, as the fused engine computes them)."""
-    from oracle import oracle
-    cores = len(os.sched_getaffinity(0))
-    field = oracle.energy_life_field((4096, 4096))
-    oracle.energy_life(field, 1, threads=cores)
-    t0 = time.perf_counter()
-    oracle.energy_life(field, steps, threads=cores)
-    elapsed = time.perf_counter() - t0
-    return {"value": 4096 * 4096 * steps / elapsed, "unit": "cell-updates/s",
-            "cores": cores, "kind": "port",
-            "sample": "EnergyLife 4096x4096 word 42, %d steps after 1 warm-up "
-                      "step, reference plain emit/absorb structure (8 buffer "
-                      "planes of 128-bit words, colours), C + OpenMP" % steps}
+def energy_cpu_baseline(steps=10):
+    """EnergyLife's generated code on the host cores, the same 4096^2
+    field, ``steps`` steps (colours included, as the GPU computes them)."""
+    from examples.energy_life import EnergyLife, EnergyLifeExperiment
+    cpu_exp = type("EnergyHost", (EnergyLifeExperiment, ),
+                   {"engine": "plain"})
+    return host_baseline(
+        EnergyLife(cpu_exp), steps,
+        "EnergyLife 4096x4096 word 42, %d steps after 1 warm-up step: the "
+        "DSL's generated emit/absorb (8 buffer planes of 128-bit words, "
+        "colours) compiled for the host, g++ -O3 -fopenmp" % steps)
 
 
 def energy_life(steps=5000, plain_steps=20, cpu=True, tiles=None,
@@ -522,13 +518,13 @@ def energy_life(steps=5000, plain_steps=20, cpu=True, tiles=None,
     engine in one graph-captured run; total energy checked before and
     after; the plain emit/absorb launch loop timed over ``plain_steps``.
     """
-    from examples.energy_life import EnergyLife, EnergyLifeExperiment
-    from oracle.oracle import energy_fields
+    from examples.energy_life import (
+        EnergyLife, EnergyLifeExperiment, field_values)
     from xentica_amd.runtime import hip
     from xentica_amd.runtime.fused import autotune
     model = EnergyLife(EnergyLifeExperiment)
     n = model.cells_num
-    total0 = int(energy_fields(model.gpu.arrays.cells.get()[:n])[0].sum())
+    total0 = int(field_values(model, 'energy').sum())
     batch, _ = autotune([model], log=log, **_tune_space(tiles, depths))
     batch.run(10)
     batch.synchronize()
@@ -539,7 +535,8 @@ def energy_life(steps=5000, plain_steps=20, cpu=True, tiles=None,
     stop.synchronize()
     ms = start.elapsed_ms(stop)
     batch.scatter()
-    energy, _, mode, _ = energy_fields(model.gpu.arrays.cells.get()[:n])
+    energy = field_values(model, 'energy')
+    mode = field_values(model, 'mode')
     value = n * steps / (ms / 1e3)
     plain_value = None
     if plain_steps:
@@ -719,7 +716,7 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
     }
 
 
-def refbench(steps=100000, plain_steps=10000):
+def refbench(steps=100000, plain_steps=10000, cpu=True, cpu_steps=200):
     """
     The reference's own benchmark (``tests/benchmark.py``: Conway's Life
     640x360, Shifting Sands, Noise TV and EvoLife through ``model.step()``,
@@ -738,6 +735,15 @@ def refbench(steps=100000, plain_steps=10000):
             out.setdefault(name, {"cells": cells})[used] = {
                 "value": speed, "unit": "cells/s",
                 "us_per_step": cells / speed * 1e6, "steps": count}
+    if cpu:
+        for name, model, experiment in benchmark.MODELS:
+            host_exp = type(experiment.__name__ + "Host", (experiment, ),
+                            {"engine": "plain"})
+            out[name]["cpu_baseline"] = host_baseline(
+                model(host_exp), cpu_steps,
+                "%d steps after 1 warm-up step: the model's generated "
+                "emit/absorb compiled for the host, g++ -O3 -fopenmp"
+                % cpu_steps)
     return out
 
 
@@ -786,8 +792,8 @@ def main():
     if args.only:
         log("%s only ..." % args.only)
         kwargs = ({"cpu": not args.no_cpu}
-                  if args.only in ("life512", "life3d", "energy", "batch")
-                  else {})
+                  if args.only in ("life512", "life3d", "energy", "batch",
+                                   "refbench") else {})
         if args.leg_steps and args.only in ("life512", "life3d", "energy",
                                             "batch"):
             kwargs["steps"] = args.leg_steps
@@ -875,7 +881,7 @@ def main():
         result["render"] = render_leg()
     if not args.no_refbench:
         log("the reference's benchmark models ...")
-        result["refbench"] = refbench()
+        result["refbench"] = refbench(cpu=not args.no_cpu)
     if not args.no_3d:
         log("3-D workload ...")
         result["life3d"] = life3d(cpu=not args.no_cpu)

@@ -130,3 +130,28 @@ class EnergyLifeSmall(EnergyLifeExperiment):
     """The same seeding on a 96x80 torus (parity tests)."""
 
     size = (96, 80)
+
+
+def field_values(model, name):
+    """The values of property ``name`` of every cell of ``model``'s main
+    plane (the container's bit layout: fields in declaration order from
+    bit 0; 128-bit words as two little-endian uint64 halves)."""
+    import numpy as np
+    shift = 0
+    for prop_name, prop in model.main.properties.items():
+        if prop_name == name:
+            width = prop.bit_width
+            break
+        shift += prop.bit_width
+    else:
+        raise KeyError(name)
+    words = model.gpu.arrays.cells.get()[:model.cells_num]
+    lo = words['lo'].astype(np.uint64)
+    hi = words['hi'].astype(np.uint64)
+    if shift >= 64:
+        val = hi >> np.uint64(shift - 64)
+    elif shift:
+        val = (lo >> np.uint64(shift)) | (hi << np.uint64(64 - shift))
+    else:
+        val = lo
+    return (val & np.uint64((1 << width) - 1)).astype(np.int64)
