@@ -1,7 +1,7 @@
 """Shared templates of the example models (``examples/base.py`` of the reference)."""
 import abc
 
-from xentica_amd import core
+from xentica import core
 
 
 class RegularCA(core.CellularAutomaton):

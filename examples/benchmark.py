@@ -19,7 +19,7 @@ from examples.game_of_life import GameOfLife, GOLExperiment
 from examples.shifting_sands import ShiftingSands, ShiftingSandsExperiment
 from examples.noisetv import NoiseTV, NoiseTVExperiment
 from examples.evolife import EvoLife, BigBangExperiment
-from xentica_amd.utils.formatters import sizeof_fmt
+from xentica.utils.formatters import sizeof_fmt
 
 MODELS = [
     ("Conway's Life", GameOfLife, GOLExperiment),

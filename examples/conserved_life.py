@@ -24,14 +24,14 @@ model runs on the plain engine (the fused engine refuses such models).
 """
 import random
 
-from xentica_amd import core
-from xentica_amd import seeds
-from xentica_amd.core import color_effects
-from xentica_amd.seeds.random import RandInt
-from xentica_amd.tools import xmath
-from xentica_amd.tools.color import GenomeColor
-from xentica_amd.tools.genetics import genome_crossover
-from xentica_amd.tools.rules import LifeLike
+from xentica import core
+from xentica import seeds
+from xentica.core import color_effects
+from xentica.seeds.random import RandInt
+from xentica.tools import xmath
+from xentica.tools.color import GenomeColor
+from xentica.tools.genetics import genome_crossover
+from xentica.tools.rules import LifeLike
 
 from examples.base import RegularCA, RegularExperiment
 

@@ -33,9 +33,9 @@ One step:
   takes that genome, with one bit flipped when the cell's generator
   (advanced once per step) says so (probability ``mutation / 65536``).
 """
-from xentica_amd import core
-from xentica_amd import seeds
-from xentica_amd.core import color_effects
+from xentica import core
+from xentica import seeds
+from xentica.core import color_effects
 
 from examples.base import RegularCA
 

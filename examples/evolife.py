@@ -9,14 +9,14 @@ apply (100 steps of :class:`CrossbreedingExperiment` hash to 2061779088,
 colours 68940598; :class:`CrossbreedingExperiment2` to 3381214984 and
 2253830262; ``tests/core/test_base.py:197-216``).
 """
-from xentica_amd import core
-from xentica_amd import seeds
-from xentica_amd.tools import xmath
-from xentica_amd.core import color_effects
-from xentica_amd.tools.color import GenomeColor
-from xentica_amd.tools.genetics import genome_crossover
-from xentica_amd.tools.rules import LifeLike
-from xentica_amd.seeds.random import RandInt
+from xentica import core
+from xentica import seeds
+from xentica.tools import xmath
+from xentica.core import color_effects
+from xentica.tools.color import GenomeColor
+from xentica.tools.genetics import genome_crossover
+from xentica.tools.rules import LifeLike
+from xentica.seeds.random import RandInt
 
 from examples.base import RegularCA, RegularExperiment
 

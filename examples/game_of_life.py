@@ -7,10 +7,10 @@ same for the reference's golden checksums to apply (for instance 1000
 steps of :class:`GameOfLife` on :class:`GOLExperiment` hash to
 2981695958, ``tests/core/test_base.py:41``).
 """
-from xentica_amd import core
-from xentica_amd import seeds
-from xentica_amd.core import color_effects
-from xentica_amd.tools.rules import LifeLike
+from xentica import core
+from xentica import seeds
+from xentica.core import color_effects
+from xentica.tools.rules import LifeLike
 
 
 class GameOfLife(core.CellularAutomaton):

@@ -7,10 +7,10 @@ mass.  Exercises ``FloatProperty`` bit-casting: 100 steps of
 :class:`GraviCowboysExperiment` hash to 1068195512
 (``tests/core/test_properties.py:97-104``).
 """
-from xentica_amd import core
-from xentica_amd import seeds
-from xentica_amd.core import color_effects
-from xentica_amd.tools import xmath
+from xentica import core
+from xentica import seeds
+from xentica.core import color_effects
+from xentica.tools import xmath
 
 from examples.base import RegularCA, RegularExperiment
 

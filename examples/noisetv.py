@@ -6,11 +6,11 @@ it again, so the generator advances twice per step.  100 steps on
 :class:`NoiseTVExperiment` hash to 2773894957
 (``tests/core/test_properties.py:64-71``).
 """
-from xentica_amd import core
-from xentica_amd import seeds
-from xentica_amd.core import color_effects
-from xentica_amd.seeds.random import RandInt
-from xentica_amd.tools import xmath
+from xentica import core
+from xentica import seeds
+from xentica.core import color_effects
+from xentica.seeds.random import RandInt
+from xentica.tools import xmath
 
 from examples.base import RegularCA, RegularExperiment
 

@@ -18,10 +18,10 @@ arrives there; the GPU tests check that.
   counts the steps since the last transfer.
 * Absorb: energy and gate offsets received are added.
 """
-from xentica_amd import core
-from xentica_amd import seeds
-from xentica_amd.core import color_effects
-from xentica_amd.tools import xmath
+from xentica import core
+from xentica import seeds
+from xentica.core import color_effects
+from xentica.tools import xmath
 
 
 class PoissonWalk(core.CellularAutomaton):

@@ -7,8 +7,8 @@ and takes the sum of what it receives, so the field moves by (-1, -1)
 every step and 16 steps move it by exactly one 16x16 block.  It is the
 exact model the neural-CA surrogate learns to predict.
 """
-from xentica_amd import core
-from xentica_amd import seeds
+from xentica import core
+from xentica import seeds
 
 from examples.shifting_sands import ShiftingSands
 

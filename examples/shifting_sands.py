@@ -6,9 +6,9 @@ sum of what it receives, so the pattern moves by (-1, -1) every step.
 1000 steps on :class:`ShiftingSandsExperiment` hash to 1639048270
 (``tests/core/test_base.py:182-189``).
 """
-from xentica_amd import core
-from xentica_amd import seeds
-from xentica_amd.core import color_effects
+from xentica import core
+from xentica import seeds
+from xentica.core import color_effects
 
 from examples.base import RegularCA, RegularExperiment
 

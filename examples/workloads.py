@@ -8,9 +8,9 @@ Benchmark and parity workloads, sized as the project's measurement plan.
 * :class:`LifeLikeRule` and :func:`batch_experiments` -- the batch of 256
   Life-like automata of uneven sizes, each with its own rule and density.
 """
-from xentica_amd import core
-from xentica_amd import seeds
-from xentica_amd.core import color_effects
+from xentica import core
+from xentica import seeds
+from xentica.core import color_effects
 
 
 class Life512Experiment(core.Experiment):
