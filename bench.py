@@ -236,16 +236,18 @@ def life512_oracle_hashes(steps):
 def render_leg(steps=8, frames=5, launches=20):
     """
     ``render()`` of a 4096x4096 Life field (viewport = field, zoom 1) after
-    ``steps`` fused steps: the whole call (kernel, device-to-host copy of the
-    int32 image, the reference's ``astype(uint8)``) and the render kernel
-    alone; the image is checked against the plain engine's.
+    ``steps`` steps: on the default engine (bits: the packed state projected
+    straight into the image) and on the fused one (colours in the int3
+    array, then the render kernel), each the whole call (kernel, device
+    cast to uint8, copy to the host); the images are checked against the
+    plain engine's.
     """
     from examples.game_of_life import GameOfLife
     from examples.workloads import Life512Experiment
     from xentica_amd.runtime import hip
     size = (4096, 4096)
     models = {}
-    for engine in ("fused", "plain"):
+    for engine in ("bits", "fused", "plain"):
         exp = type("Render4k" + engine.title(), (Life512Experiment, ),
                    {"size": size, "engine": engine})
         model = GameOfLife(exp)
@@ -253,34 +255,42 @@ def render_leg(steps=8, frames=5, launches=20):
         for _ in range(steps):
             model.step()
         models[engine] = model
-    model = models["fused"]
-    image = model.render()
-    same = bool(np.array_equal(image, models["plain"].render()))
-    hip.device().synchronize()
-    t0 = time.perf_counter()
-    for _ in range(frames):
-        model.render()
-    call_ms = (time.perf_counter() - t0) / frames * 1e3
+    want = models["plain"].render()
+    out = {"workload": "render() of Life 4096x4096 after %d steps, viewport "
+                       "4096x4096, zoom 1" % steps}
     pixels = size[0] * size[1]
-    args = (model.renderer.get_args_vals(model)
-            + [param.dtype(param.value) for param in model._render_params]
-            + [np.int64(pixels)])
-    kernel = model.gpu.kernels.render
+    for engine in ("bits", "fused"):
+        model = models[engine]
+        same = bool(np.array_equal(model.render(), want))
+        hip.device().synchronize()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            model.render()
+        call_ms = (time.perf_counter() - t0) / frames * 1e3
+        out[engine] = {"frame_ms": call_ms, "frames_per_s": 1e3 / call_ms,
+                       "image_matches_plain_engine": same}
+    # the kernels alone
+    bits = models["bits"]
+    params = [param.dtype(param.value) for param in bits._render_params]
     hip.device().synchronize()
     t0 = time.perf_counter()
     for _ in range(launches):
-        model._engine.launch_elementwise(kernel, args, pixels)
+        bits._engine.render_frame(params, pixels)
     hip.device().synchronize()
-    kernel_us = (time.perf_counter() - t0) / launches * 1e6
-    return {
-        "workload": "render() of Life 4096x4096 after %d fused steps, "
-                    "viewport 4096x4096, zoom 1" % steps,
-        "frame_ms": call_ms, "frames_per_s": 1e3 / call_ms,
-        "kernel_us": kernel_us,
-        "kernel_GBps": pixels * 24 / (kernel_us * 1e-6) / 1e9,
-        "kernel_bytes_per_pixel": 24,
-        "image_matches_plain_engine": same,
-    }
+    out["bits"]["kernel_us"] = (time.perf_counter() - t0) / launches * 1e6
+    out["bits"]["kernel_bytes_per_pixel"] = 12
+    fused = models["fused"]
+    args = (fused.renderer.get_args_vals(fused) + params
+            + [np.int64(pixels)])
+    kernel = fused.gpu.kernels.render
+    hip.device().synchronize()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        fused._engine.launch_elementwise(kernel, args, pixels)
+    hip.device().synchronize()
+    out["fused"]["kernel_us"] = (time.perf_counter() - t0) / launches * 1e6
+    out["fused"]["kernel_bytes_per_pixel"] = 24
+    return out
 
 
 def batched(steps=1000, plain_steps=10, cpu=True):

@@ -193,3 +193,35 @@ def test_headline_through_model_step():
         want = kernels.crc32_u8(plain.gpu.arrays.cells.device_ptr(),
                                 plain.cells_num)
         assert model._engine.crc32() == want, "after %d" % (250 * block + 250)
+
+
+@pytest.mark.parametrize("zoom,pos,viewport", [(1, [0, 0], (640, 360)),
+                                               (3, [17, 5], (200, 120)),
+                                               (2, [630, 350], (333, 77))])
+def test_render_from_packed_state_vs_plain(zoom, pos, viewport):
+    """``render()`` on the bits engine projects the packed state directly
+    (no colour expansion) and gives the plain engine's image, for zooms,
+    pans across the torus edge and viewports smaller than the field."""
+    exps = [type("Render" + name, (with_engine(GOLExperiment2, name), ),
+                 {"zoom": zoom, "pos": list(pos)}) for name in ("plain",
+                                                                "bits")]
+    plain, bits = GameOfLife(exps[0]), GameOfLife(exps[1])
+    for model in (plain, bits):
+        model.set_viewport(viewport)
+        run(model, 21)
+    assert bits._engine._stale  # the packed state is the current one
+    assert np.array_equal(bits.render(), plain.render())
+    assert bits._engine._stale  # rendering did not expand it
+
+
+def test_render_3d_projection_vs_plain():
+    class Small(Life3DExperiment):
+        size = (64, 48, 20)
+        zoom = 1
+        pos = [0, 0, 0]
+    plain = LifeLike3D(with_engine(Small, "plain"))
+    bits = LifeLike3D(with_engine(Small, "bits"))
+    for model in (plain, bits):
+        model.set_viewport((64, 48))
+        run(model, 5)
+    assert np.array_equal(bits.render(), plain.render())

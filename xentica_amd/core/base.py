@@ -592,12 +592,18 @@ class CellularAutomaton(Translator, metaclass=BSCA):
             msg = "Viewport is not set, call set_viewport() before rendering."
             raise XenticaException(msg)
         with self._lock:
-            self._engine.flush()
-            args = self.renderer.get_args_vals(self)
-            args += [param.dtype(param.value) for param in self._render_params]
-            args.append(np.int64(self.width * self.height))
-            self._engine.launch_elementwise(
-                self.gpu.kernels.render, args, self.width * self.height)
+            pixels = self.width * self.height
+            render_params = [param.dtype(param.value)
+                             for param in self._render_params]
+            # an engine that keeps the state in its own form may project it
+            # into the image itself (the bits engine reads the packed cells)
+            direct = getattr(self._engine, "render_frame", None)
+            if direct is None or not direct(render_params, pixels):
+                self._engine.flush()
+                args = self.renderer.get_args_vals(self) + render_params
+                args.append(np.int64(pixels))
+                self._engine.launch_elementwise(
+                    self.gpu.kernels.render, args, pixels)
             # the reference's img.get().astype(uint8), with the cast on the
             # device: a quarter of the bytes cross to the host
             from xentica_amd.runtime import hip, kernels

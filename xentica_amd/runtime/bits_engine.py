@@ -126,6 +126,60 @@ class BitsEngine:
         arrays.colors.set(colors)
         self._packed = False
 
+    def render_frame(self, render_params, pixels):
+        """
+        Project the packed state into the model's image with the model's
+        own render body (``renderers.py:220-270``), its colour load replaced
+        by the colour of the cell's bit: the colours are never expanded.
+        Returns False (the caller renders from the arrays) when the packed
+        state is not the current one or the render body is not the plain
+        renderer's.
+        """
+        if self._pending:
+            self._launch()
+        if not (self._packed and self._stale):
+            return False
+        function = self._render_function()
+        if function is None:
+            return False
+        model = self.model
+        _, _, (dead, alive) = self._pending_rule
+        args = model.renderer.get_args_vals(model)
+        args[0] = np.uint64(self._bufs[self._cur].ptr)
+        args += list(render_params)
+        args += [np.int32(v) for v in dead + alive]
+        args.append(np.int64(pixels))
+        self.launch_elementwise(function, args, pixels)
+        return True
+
+    _COLOUR_LOAD = "int3 c = col[ii];\n"
+
+    def _render_function(self):
+        if getattr(self, "_render_fn", False) is not False:
+            return self._render_fn
+        self._render_fn = None
+        model = self.model
+        body = model.kernel_bodies.get('render', "")
+        if body.count(self._COLOUR_LOAD) != 1 or model.renderer.args[0] != \
+                ("int3", "*col"):
+            return None
+        from xentica_amd.core.base import _elementwise_kernel
+        body = body.replace(self._COLOUR_LOAD, (
+            "int3 c = ((_bits[(unsigned) ii >> 5] >> (ii & 31)) & 1u) ?\n"
+            "    make_int3(_a0, _a1, _a2) : make_int3(_d0, _d1, _d2);\n"))
+        args = [("const unsigned", "*_bits")] + list(model.renderer.args[1:])
+        args += [(p.ctype, p.name) for p in model._render_params]
+        args += [("int", name) for name in
+                 ("_d0", "_d1", "_d2", "_a0", "_a1", "_a2")]
+        source = model.build_defines() + _elementwise_kernel(
+            "render_bits", args, body)
+        module = hip.Module(hip.compile_source(
+            source, "bsca_render_bits.hip",
+            hip.compile_options(model.fp_contract)))
+        self._render_module = module
+        self._render_fn = module.get_function("render_bits")
+        return self._render_fn
+
     def invalidate(self):
         """The arrays were written directly (a batch's ``scatter()``): the
         next step packs them again."""

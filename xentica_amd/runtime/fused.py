@@ -1065,6 +1065,34 @@ class Batch:
                 model._engine.invalidate()
         self.stream.synchronize()
 
+    def save(self, filename):
+        """Write every instance's state in the reference's checkpoint
+        format (``xentica/core/base.py:755-763``): a pickled
+        ``{"instances": [{"cells", "colors", "random"}, ...]}`` in the
+        order of :attr:`models`; each entry is what that model's own
+        ``save()`` writes."""
+        from xentica_amd.runtime import checkpoint
+        self.synchronize()
+        self.scatter()
+        checkpoint.write(filename, {"instances": [
+            {"cells": model.gpu.arrays.cells.get(),
+             "colors": model.gpu.arrays.colors.get(),
+             "random": model.random} for model in self.models]})
+
+    def load(self, filename):
+        """Read a checkpoint written by :meth:`save` into the models and
+        the batch (``xentica/core/base.py:765-771`` per instance)."""
+        from xentica_amd.runtime import checkpoint
+        states = checkpoint.read(filename)["instances"]
+        if len(states) != len(self.models):
+            raise XenticaException("checkpoint holds %d instances, the batch "
+                                   "%d" % (len(states), len(self.models)))
+        self.synchronize()
+        for model, state in zip(self.models, states):
+            model._engine.load_state(state["cells"], state["colors"])
+            model.random.load(state["random"])
+        self.gather()
+
     def _launch(self, steps, fast):
         function = (self.program.function_fast if fast
                     else self.program.function)

@@ -12,7 +12,7 @@ rule is the reference's Life-like rule (``examples/game_of_life.py:55-70,
 import numpy as np
 
 from xentica_amd.core.exceptions import XenticaException
-from xentica_amd.runtime import hip, kernels
+from xentica_amd.runtime import checkpoint, hip, kernels
 
 __all__ = ['LifeBitsField', 'Life3DField', 'Life3DBytes', ]
 
@@ -111,6 +111,19 @@ class LifeBitsField:
         if in_b:
             self._cur = 1 - self._cur
 
+    def save(self, filename):
+        """Write the state in the reference's checkpoint format: plane 0 of
+        a uint8 cells array with the 2-D Moore buffer planes (zero), zero
+        colours and a fresh random stream (``xentica/core/base.py:755-763``);
+        a ``GameOfLife`` of the same size loads it."""
+        checkpoint.write(filename, checkpoint.field_state(
+            self.get_cells(), checkpoint.MOORE_BUFFERS[2]))
+
+    def load(self, filename):
+        """Read plane 0 of a checkpoint (this field's or a model's)."""
+        self.set_cells(checkpoint.main_plane(checkpoint.read(filename),
+                                             self.n_cells))
+
     def synchronize(self):
         """Wait for queued work."""
         if self.stream is not None:
@@ -180,6 +193,18 @@ class Life3DField:
                 self.born, self.sustain, self.z_tiles, self.stream)
             self._cur = 1 - self._cur
 
+    def save(self, filename):
+        """Write the state in the reference's checkpoint format (plane 0
+        of a uint8 cells array with the 26 buffer planes of the 3-D Moore
+        list, zero; zero colours; a fresh random stream)."""
+        checkpoint.write(filename, checkpoint.field_state(
+            self.get_cells(), checkpoint.MOORE_BUFFERS[3]))
+
+    def load(self, filename):
+        """Read plane 0 of a checkpoint (this field's or a model's)."""
+        self.set_cells(checkpoint.main_plane(checkpoint.read(filename),
+                                             self.n_cells))
+
     def synchronize(self):
         """Wait for queued work."""
         if self.stream is not None:
@@ -244,6 +269,17 @@ class Life3DBytes:
     def crc32(self):
         """``binascii.crc32`` of the plane, on the device."""
         return kernels.crc32_u8(self._plane(), self.n_cells, self.stream)
+
+    def save(self, filename):
+        """Write the state in the reference's checkpoint format (as
+        :meth:`Life3DField.save`)."""
+        checkpoint.write(filename, checkpoint.field_state(
+            self.get_cells(), checkpoint.MOORE_BUFFERS[3]))
+
+    def load(self, filename):
+        """Read plane 0 of a checkpoint (this field's or a model's)."""
+        self.set_cells(checkpoint.main_plane(checkpoint.read(filename),
+                                             self.n_cells))
 
     def step(self, steps=1):
         """Advance ``steps`` steps (asynchronous)."""
