@@ -605,15 +605,18 @@ def _event_ms(stream, work, repeats):
 
 
 def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
-              eval_word=199, train_steps=800, repeats=20):
+              eval_word=199, train_steps=1500, repeats=20):
     """
-    The neural-CA surrogate on 4096x4096 sand heights (0-8, word 42
-    family): trained on exact-engine pairs (state at t, state at t + 16),
-    scored on a held-out field, and timed to the coarse field (256x256
-    block means 16 steps ahead) against the exact engine.
+    The neural-CA surrogate on 4096x4096 sliding sand (heights 0-8, word
+    42 family; ``examples/sand_slide.py``: half a block of drift in 16
+    steps and slides down steep edges): trained on exact-engine pairs
+    (state at t, state at t + 16), scored on a held-out field against the
+    exact engine, persistence and the best one-block roll, and timed to the
+    coarse field (256x256 block means 16 steps ahead) against the exact
+    engine.
     """
     import torch
-    from examples.sand_heights import SandHeights, sand_experiment
+    from examples.sand_slide import SandSlide, slide_experiment
     from xentica_amd.runtime.fused import autotune
     from xentica_amd.surrogate.network import (
         NCASurrogate, NCAFunction, encode, forward_raw, predict)
@@ -621,12 +624,12 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
         pairs_from_model, train, evaluate, value_mask)
     size = (4096, 4096)
     h, w = size[1] // 16, size[0] // 16
-    data = [pairs_from_model(SandHeights(sand_experiment(word, size)), pairs)
+    data = [pairs_from_model(SandSlide(slide_experiment(word, size)), pairs)
             for word in train_words]
     inputs = torch.cat([d[0] for d in data])
     targets = torch.cat([d[1] for d in data])
     eval_in, eval_out = pairs_from_model(
-        SandHeights(sand_experiment(eval_word, size)), 2)
+        SandSlide(slide_experiment(eval_word, size)), 2)
     net = NCASurrogate(seed=0)
     t0 = time.perf_counter()
     losses = train(net, inputs, targets, steps=train_steps, batch=4)
@@ -636,7 +639,7 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
 
     # time to the coarse field: exact engine (fused, autotuned, graph) for
     # 16 steps + block means, against encoder + surrogate forward
-    model = SandHeights(sand_experiment(eval_word, size))
+    model = SandSlide(slide_experiment(eval_word, size))
     mask = value_mask(model)
     batch, _ = autotune([model], trial_steps=16)
     stream = batch.stream
@@ -664,8 +667,8 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
     fast_ms = _event_ms(stream, fast, repeats)
 
     # the reference-structured plain engine for the same 16 steps
-    plain = SandHeights(type("SandPlain", (sand_experiment(eval_word, size),),
-                             {"engine": "plain"}))
+    plain = SandSlide(type("SandPlain", (slide_experiment(eval_word, size),),
+                           {"engine": "plain"}))
     plain.step()
     hip_sync = __import__("xentica_amd.runtime.hip", fromlist=["hip"])
     hip_sync.device().synchronize()
@@ -697,9 +700,11 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
     mfma_flop = 2 * (16 * 64 + 64 * 16)
     bytes_cell = 16 * 4 * 2
     return {
-        "workload": "neural-CA surrogate on 4096x4096 sand heights 0-8: 3x3 "
-                    "depthwise (16 ch) + 1x1 MLP 16-64-16, bf16 MFMA / fp32 "
-                    "accumulate; predicts 256x256 block means 16 steps ahead",
+        "workload": "neural-CA surrogate on 4096x4096 sliding sand "
+                    "(heights 0-8, half a block of drift per 16 steps, "
+                    "slides down steep edges): 3x3 depthwise (16 ch) + 1x1 "
+                    "MLP 16-64-16, bf16 MFMA / fp32 accumulate; predicts "
+                    "256x256 block means 16 steps ahead",
         "train": {"pairs": int(inputs.shape[0]), "steps": train_steps,
                   "seconds": train_s, "loss_first": losses[0],
                   "loss_last": losses[-1]},

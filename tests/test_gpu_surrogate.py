@@ -211,3 +211,27 @@ def test_predict_and_forward_at_the_workload_size():
     want_y, _ = nca.forward(x.cpu().numpy(), oracle_params, rounding=True)
     assert np.abs(y.cpu().numpy() - want_y).max() <= ATOL_Y
     assert np.abs(pred.cpu().numpy() - want_y.mean(axis=-1)).max() <= ATOL_Y
+
+
+def test_training_on_sliding_sand():
+    """
+    The sliding-sand rule (half a block of drift in 16 steps plus slides
+    down steep edges): after training on exact-engine pairs, the held-out
+    block-mean L1 is at most 2 % of the mean height and at most a fifth of
+    both methods without the network, persistence and the best one-block
+    roll of the coarse field.
+    """
+    from examples.sand_slide import SandSlide, slide_experiment
+    pairs = [pairs_from_model(SandSlide(slide_experiment(w, (1024, 1024))),
+                              3) for w in (31, 32, 33, 34, 35, 36)]
+    inputs = torch.cat([p[0] for p in pairs])
+    targets = torch.cat([p[1] for p in pairs])
+    held_in, held_out = pairs_from_model(
+        SandSlide(slide_experiment(98, (1024, 1024))), 3)
+    net = NCASurrogate(seed=0)
+    train(net, inputs, targets, steps=1500, batch=4)
+    after = evaluate(net, held_in, held_out)
+    print("sliding sand held out:", after, flush=True)
+    assert after["l1_relative"] <= 0.02
+    assert 5 * after["l1"] <= after["persistence_l1"]
+    assert 5 * after["l1"] <= after["roll_l1"]

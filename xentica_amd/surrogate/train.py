@@ -93,8 +93,11 @@ def train(net, inputs, targets, steps=600, batch=4, lr=1e-2, seed=0,
 def evaluate(net, inputs, targets):
     """
     L1 of the block-mean observable against the exact engine, absolute
-    and relative to the mean exact block height, for the surrogate and
-    for persistence.
+    and relative to the mean exact block height, for the surrogate and for
+    the two methods without a network: persistence (the observable at step
+    t) and the best of the eight one-block rolls of it (the whole coarse
+    field moved by one block, the exact answer for a rule that translates
+    the field by one block in the horizon).
     """
     with torch.no_grad():
         exact = targets.mean(dim=-1)
@@ -103,9 +106,18 @@ def evaluate(net, inputs, targets):
         scale = float(exact.abs().mean())
         l1_net = float((pred - exact).abs().mean())
         l1_persist = float((persist - exact).abs().mean())
+        rolls = {}
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                if dy or dx:
+                    moved = torch.roll(persist, shifts=(dy, dx), dims=(1, 2))
+                    rolls[(dy, dx)] = float((moved - exact).abs().mean())
+        best = min(rolls, key=rolls.get)
     return {
         "l1": l1_net, "l1_relative": l1_net / scale,
         "persistence_l1": l1_persist,
         "persistence_l1_relative": l1_persist / scale,
+        "roll_l1": rolls[best], "roll_l1_relative": rolls[best] / scale,
+        "roll_blocks": list(best),
         "mean_block_height": scale,
     }
