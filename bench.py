@@ -293,7 +293,7 @@ def render_leg(steps=8, frames=5, launches=20):
     return out
 
 
-def batched(steps=1000, plain_steps=10, cpu=True):
+def batched(steps=1000, plain_steps=10, cpu=True, depth=0, strip_rows=0):
     """
     The batch workload: 256 Life-like tori, sides 64-2048, own rules.
 
@@ -311,7 +311,12 @@ def batched(steps=1000, plain_steps=10, cpu=True):
     exps = batch_experiments()
     models = [LifeLikeRule(exp) for exp in exps]
     total = sum(model.cells_num for model in models)
-    batch, tuned = bits_autotune(models, log=log)
+    if depth:
+        from xentica_amd.runtime.bits_batch import BitsBatch
+        batch, tuned = BitsBatch(models, depth=depth,
+                                 strip_rows=strip_rows or 48), {}
+    else:
+        batch, tuned = bits_autotune(models, log=log)
 
     def timed_run(item):
         item.run(steps)                       # capture, warm up
@@ -797,6 +802,12 @@ def main():
     parser.add_argument("--fused-depth", type=int, default=0,
                         help="the fused legs use this depth instead of "
                              "autotuning it")
+    parser.add_argument("--bits-depth", type=int, default=0,
+                        help="the bit-sliced batch runs this depth instead "
+                             "of autotuning (with --only batch)")
+    parser.add_argument("--bits-strips", type=int, default=0,
+                        help="strip rows of the bit-sliced batch with "
+                             "--bits-depth (0: 48)")
     parser.add_argument("--only", choices=sorted(_LEGS),
                         help="run one secondary workload alone (profiling)")
     args = parser.parse_args()
@@ -818,6 +829,9 @@ def main():
                                          args.fused_tile.split("x")), )
             if args.fused_depth:
                 kwargs["depths"] = (args.fused_depth, )
+        if args.only == "batch" and args.bits_depth:
+            kwargs["depth"] = args.bits_depth
+            kwargs["strip_rows"] = args.bits_strips
         if args.only in ("energy", "batch") and args.no_plain:
             kwargs["plain_steps"] = 0
         print(json.dumps({args.only: _LEGS[args.only](**kwargs)}), flush=True)

@@ -18,12 +18,20 @@
 //   p   = bdw + sum_t dw[:, t] * x[neighbour t]        (fp32, taps in order)
 //   z1  = b1 + W1 . bf16(p)                              (MFMA, fp32 acc)
 //   y   = b2 + W2 . bf16(relu(z1))                       (MFMA, fp32 acc)
-// The two layers use v_mfma_f32_16x16x16_bf16 on 16 cells at a time in the
-// transposed form  Z1^T = W1 . P^T  and  Y^T = W2 . H^T:  with operands
-// A[m][k] in lane (m, k/4) and B[k][n] and D[m][n] in lane (n, m/4), the
-// D tile of a 16-hidden-unit block of layer 1 is already the B operand
-// of the matching K block of layer 2, so no data moves between lanes.
-// Each lane owns one cell (lane % 16) and four channels (4 * (lane / 16)).
+// The two layers run on 16 cells at a time in the transposed form
+// Z1^T = W1 . P^T and Y^T = W2 . H^T.  Layer 1 contracts over the 16
+// input channels: v_mfma_f32_16x16x16_bf16, A[m][k] in lane (m, k/4),
+// B[k][n] and D[m][n] in lane (n, m/4), one instruction per block of 16
+// hidden units.  The D tile of a hidden block is then already laid out as
+// a B operand: lane (n, g) holds hidden units 16 b + 4g .. +3.  Layer 2
+// contracts over the 64 hidden units with gfx950's
+// v_mfma_f32_16x16x32_bf16 (A[m][k] in lane (m, k/8), B[k][n] in lane
+// (n, k/8)), two hidden blocks per instruction: the order of k inside a
+// contraction is free, so lane g's eight k values are taken as the four
+// units of block 2j followed by the four of block 2j + 1 -- exactly what
+// the lane holds -- and W2's A operand is loaded in the same order.  No
+// data moves between lanes.  Each lane owns one cell (lane % 16) and four
+// channels (4 * (lane / 16)).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -52,6 +60,8 @@ constexpr int kParams = kOffB2 + kC;
 static_assert(kParams == XEN_NCA_PARAMS, "parameter layout");
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mfma_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // fp32 -> bf16 bits, round to nearest even (inputs are finite)
@@ -63,6 +73,17 @@ __device__ __forceinline__ short to_bf16(float f) {
 
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// D += A . B over K = 32 (v_mfma_f32_16x16x32_bf16): a lane's eight k
+// values are the four of a_lo / b_lo followed by the four of a_hi / b_hi.
+__device__ __forceinline__ f32x4 mfma32(bf16x4 a_lo, bf16x4 a_hi, bf16x4 b_lo, bf16x4 b_hi,
+                                        f32x4 c) {
+    const bf16x8 a = __builtin_shufflevector(a_lo, a_hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    const bf16x8 b = __builtin_shufflevector(b_lo, b_hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mfma_bf16x8, a),
+                                                   __builtin_bit_cast(mfma_bf16x8, b), c, 0,
+                                                   0, 0);
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -180,13 +201,19 @@ __device__ __forceinline__ f32x4 mlp(bf16x4 pb, int g, int m, const short *__res
                                      const float *__restrict__ b2) {
     f32x4 out = load_bias(b2, 4 * g);
 #pragma unroll
-    for (int blk = 0; blk < 4; ++blk) {
-        const bf16x4 a1 = *reinterpret_cast<const bf16x4 *>(w1 + (16 * blk + m) * kC + 4 * g);
-        const f32x4 z = mfma(a1, pb, load_bias(b1, 16 * blk + 4 * g));
-        const bf16x4 hb = pack4(fmaxf(z[0], 0.f), fmaxf(z[1], 0.f), fmaxf(z[2], 0.f),
-                                fmaxf(z[3], 0.f));
-        const bf16x4 a2 = *reinterpret_cast<const bf16x4 *>(w2 + m * kHid + 16 * blk + 4 * g);
-        out = mfma(a2, hb, out);
+    for (int pair = 0; pair < 2; ++pair) {
+        bf16x4 hb[2], a2[2];
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int blk = 2 * pair + half;
+            const bf16x4 a1 =
+                *reinterpret_cast<const bf16x4 *>(w1 + (16 * blk + m) * kC + 4 * g);
+            const f32x4 z = mfma(a1, pb, load_bias(b1, 16 * blk + 4 * g));
+            hb[half] = pack4(fmaxf(z[0], 0.f), fmaxf(z[1], 0.f), fmaxf(z[2], 0.f),
+                             fmaxf(z[3], 0.f));
+            a2[half] = *reinterpret_cast<const bf16x4 *>(w2 + m * kHid + 16 * blk + 4 * g);
+        }
+        out = mfma32(a2[0], a2[1], hb[0], hb[1], out);
     }
     return out;
 }
@@ -356,6 +383,7 @@ __global__ __launch_bounds__(kBlock) void backward_kernel(
         g_b2[2] += d4.z;
         g_b2[3] += d4.w;
         f32x4 dp = {0.f, 0.f, 0.f, 0.f};
+        bf16x4 a1t_lo, dzb_lo;  // the even block's operands, until the odd one
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             t.dot[(4 * g + i) * 16 + m] = db[i];
@@ -379,11 +407,15 @@ __global__ __launch_bounds__(kBlock) void backward_kernel(
                 g_b1[blk][i] += dz[i];
             }
             const bf16x4 dzb = pack4(dz[0], dz[1], dz[2], dz[3]);
-            // dP^T += W1[block]^T . dZ1^T: A[m=in][k=hidden] = W1[16 blk + k][m]
+            // dP^T += W1[blocks]^T . dZ1^T over two hidden blocks at a time
+            // (K = 32): A[m=in][k=hidden] = W1[16 blk + k][m]
             bf16x4 a1t;
 #pragma unroll
             for (int i = 0; i < 4; ++i) a1t[i] = w1[(16 * blk + 4 * g + i) * kC + m];
-            dp = mfma(a1t, dzb, dp);
+            if (blk & 1)
+                dp = mfma32(a1t_lo, a1t, dzb_lo, dzb, dp);
+            a1t_lo = a1t;
+            dzb_lo = dzb;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 t.ht[(16 * blk + 4 * g + i) * 16 + m] = hb[i];
