@@ -233,63 +233,83 @@ def life512_oracle_hashes(steps):
     return [int(h) for h in want[1:]]
 
 
-def render_leg(steps=8, frames=5, launches=20):
+def render_leg(steps=7, frames=5, launches=20):
     """
-    ``render()`` of a 4096x4096 Life field (viewport = field, zoom 1) after
-    ``steps`` steps: on the default engine (bits: the packed state projected
-    straight into the image) and on the fused one (colours in the int3
-    array, then the render kernel), each the whole call (kernel, device
-    cast to uint8, copy to the host); the images are checked against the
-    plain engine's.
+    Frames of a 4096x4096 Life field (viewport = field, zoom 1): ``steps``
+    calls of ``model.step()``, then ``render()``.  Engines: bits (the
+    packed state projected straight into the image), fused with the render
+    epilogue (the queued steps' last launch writes the image from the
+    colours in LDS) and fused with the steps flushed first (colours to the
+    int3 array, then the render kernel).  Each frame's image is checked
+    against the plain engine's; timed: the whole call (kernels, device cast
+    to uint8, copy to the host) and the device part alone (steps plus image
+    kernels, HIP events).
     """
     from examples.game_of_life import GameOfLife
     from examples.workloads import Life512Experiment
     from xentica_amd.runtime import hip
     size = (4096, 4096)
+    pixels = size[0] * size[1]
     models = {}
-    for engine in ("bits", "fused", "plain"):
+    for name, engine in (("bits", "bits"), ("fused_epilogue", "fused"),
+                         ("fused_flush_then_render", "fused"),
+                         ("plain", "plain")):
         exp = type("Render4k" + engine.title(), (Life512Experiment, ),
                    {"size": size, "engine": engine})
         model = GameOfLife(exp)
         model.set_viewport(size)
+        models[name] = model
+
+    def frame(name):
+        model = models[name]
         for _ in range(steps):
             model.step()
-        models[engine] = model
-    want = models["plain"].render()
-    out = {"workload": "render() of Life 4096x4096 after %d steps, viewport "
-                       "4096x4096, zoom 1" % steps}
-    pixels = size[0] * size[1]
-    for engine in ("bits", "fused"):
-        model = models[engine]
-        same = bool(np.array_equal(model.render(), want))
+        if name == "fused_flush_then_render":
+            model._engine.flush()
+        return model.render()
+
+    def device_part(name):
+        model = models[name]
+        params = [param.dtype(param.value) for param in model._render_params]
+        for _ in range(steps):
+            model.step()
+        if name == "fused_flush_then_render":
+            model._engine.flush()
+            args = (model.renderer.get_args_vals(model) + params
+                    + [np.int64(pixels)])
+            model._engine.launch_elementwise(model.gpu.kernels.render, args,
+                                             pixels)
+        else:
+            model._engine.render_frame(params, pixels)
+
+    out = {"workload": "frames of Life 4096x4096: %d model.step() calls, "
+                       "then render(); viewport 4096x4096, zoom 1" % steps}
+    names = ("bits", "fused_epilogue", "fused_flush_then_render")
+    for name in names:
+        out[name] = {"images_match_plain_engine": True}
+    for _ in range(2):
+        want = frame("plain")
+        for name in names:
+            if not np.array_equal(frame(name), want):
+                out[name]["images_match_plain_engine"] = False
+    out["fused_epilogue"]["epilogue_frames"] = \
+        models["fused_epilogue"]._engine.epilogue_frames
+    for name in names:
         hip.device().synchronize()
         t0 = time.perf_counter()
         for _ in range(frames):
-            model.render()
-        call_ms = (time.perf_counter() - t0) / frames * 1e3
-        out[engine] = {"frame_ms": call_ms, "frames_per_s": 1e3 / call_ms,
-                       "image_matches_plain_engine": same}
-    # the kernels alone
-    bits = models["bits"]
-    params = [param.dtype(param.value) for param in bits._render_params]
-    hip.device().synchronize()
-    t0 = time.perf_counter()
-    for _ in range(launches):
-        bits._engine.render_frame(params, pixels)
-    hip.device().synchronize()
-    out["bits"]["kernel_us"] = (time.perf_counter() - t0) / launches * 1e6
-    out["bits"]["kernel_bytes_per_pixel"] = 12
-    fused = models["fused"]
-    args = (fused.renderer.get_args_vals(fused) + params
-            + [np.int64(pixels)])
-    kernel = fused.gpu.kernels.render
-    hip.device().synchronize()
-    t0 = time.perf_counter()
-    for _ in range(launches):
-        fused._engine.launch_elementwise(kernel, args, pixels)
-    hip.device().synchronize()
-    out["fused"]["kernel_us"] = (time.perf_counter() - t0) / launches * 1e6
-    out["fused"]["kernel_bytes_per_pixel"] = 24
+            frame(name)
+        out[name]["frame_ms"] = (time.perf_counter() - t0) / frames * 1e3
+        start, stop = hip.Event(), hip.Event()
+        device_part(name)
+        hip.device().synchronize()
+        start.record(None)
+        for _ in range(launches):
+            device_part(name)
+        stop.record(None)
+        stop.synchronize()
+        out[name]["device_us_per_frame"] = \
+            start.elapsed_ms(stop) / launches * 1e3
     return out
 
 
@@ -610,7 +630,7 @@ def _event_ms(stream, work, repeats):
 
 
 def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
-              eval_word=199, train_steps=1500, repeats=20):
+              eval_word=199, train_steps=1500, repeats=20, cpu=True):
     """
     The neural-CA surrogate on 4096x4096 sliding sand (heights 0-8, word
     42 family; ``examples/sand_slide.py``: half a block of drift in 16
@@ -733,7 +753,41 @@ def surrogate(train_words=(101, 102, 103, 104, 105, 106), pairs=3,
             "hbm_frac": bytes_cell * cells / (fwd_ms / 1e3) / 1e9
                         / HBM_PEAK_GBS},
         "forward_backward_ms": fb_ms,
+        **({"cpu_baseline": surrogate_cpu_baseline(eval_word, size, mask)}
+           if cpu else {}),
     }
+
+
+def surrogate_cpu_baseline(word, size, mask, horizon=16):
+    """
+    Time to the coarse field on the host: the sliding-sand model's
+    generated emit/absorb compiled for the host cores runs ``horizon``
+    steps of the held-out field, then NumPy takes the 16x16 block means
+    of the heights.
+    """
+    from examples.sand_slide import SandSlide, slide_experiment
+    from xentica_amd.runtime.hostc import PlainHost, host_threads
+    model = SandSlide(type("SandHost", (slide_experiment(word, size), ),
+                           {"engine": "plain"}))
+    host = PlainHost(model)
+    host.load(model.gpu.arrays.cells.get(), model.gpu.arrays.colors.get())
+    cores = host_threads()
+    host.step(1, threads=cores)  # warm the pages and the thread pool
+    host.load(model.gpu.arrays.cells.get(), model.gpu.arrays.colors.get())
+    t0 = time.perf_counter()
+    host.step(horizon, threads=cores)
+    grains = (host.cells & mask).reshape(size[1] // 16, 16, size[0] // 16,
+                                         16)
+    means = grains.mean(axis=(1, 3), dtype=np.float32)
+    elapsed = time.perf_counter() - t0
+    return {"value": elapsed * 1e3, "unit": "ms to the coarse field",
+            "cores": cores, "kind": "port",
+            "cell_updates_per_s": model.cells_num * horizon / elapsed,
+            "coarse_mean": float(means.mean()),
+            "sample": "the whole held-out workload: %d steps of 4096x4096 "
+                      "sliding sand (the DSL's generated emit/absorb, 8 "
+                      "buffer planes, g++ -O3 -fopenmp) and the 256x256 "
+                      "block means" % horizon}
 
 
 def refbench(steps=100000, plain_steps=10000, cpu=True, cpu_steps=200):
@@ -819,7 +873,7 @@ def main():
         log("%s only ..." % args.only)
         kwargs = ({"cpu": not args.no_cpu}
                   if args.only in ("life512", "life3d", "energy", "batch",
-                                   "refbench") else {})
+                                   "refbench", "surrogate") else {})
         if args.leg_steps and args.only in ("life512", "life3d", "energy",
                                             "batch"):
             kwargs["steps"] = args.leg_steps
@@ -922,7 +976,7 @@ def main():
         result["batched"] = batched(cpu=not args.no_cpu)
     if not args.no_surrogate:
         log("surrogate ...")
-        result["surrogate"] = surrogate()
+        result["surrogate"] = surrogate(cpu=not args.no_cpu)
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline()

@@ -46,7 +46,7 @@ from xentica_amd.core.topology.neighborhood import OrthogonalNeighborhood
 from xentica_amd.runtime import hip, ranges
 
 __all__ = ['FusedProgram', 'FusedEngine', 'Batch', 'eligible', 'autotune',
-           'TUNE_TILES', 'TUNE_DEPTHS']
+           'render_tables', 'TUNE_TILES', 'TUNE_DEPTHS']
 
 
 class _DoesNotFit(XenticaException):
@@ -148,6 +148,8 @@ class FusedProgram:
                 "fused engine: tile %s at depth %d needs %d bytes of LDS" % (
                     self.tile, self.depth, self.lds_bytes))
         self.inst_dtype = self._instance_dtype()
+        self.width_prefix = model.topology.lattice.width_prefix
+        self._render_fns, self._render_modules = {}, {}
         defines = "".join(
             const.get_define_code() for name, const in model.constants.items()
             if not name.startswith(model.topology.lattice.width_prefix))
@@ -174,6 +176,8 @@ class FusedProgram:
                            + access.color_code + "}\n" + access.store_code)
         self.source = self._render(defines, emit_body, absorb_body, None,
                                    margin)
+        # the pieces of each form, for the render-epilogue variants
+        self._forms = {False: (defines, emit_body, absorb_body, None, margin)}
         # the form that computes the colour on a launch's last step only,
         # without reading the stored colour: each step is held twice, with
         # the colour and without it
@@ -184,6 +188,11 @@ class FusedProgram:
             memory = type(model).color.memory
             color_code = access.color_code.replace(memory, "") if memory \
                 else access.color_code
+            self._forms[True] = (
+                defines, emit_body,
+                absorb_code + "if (_inner) {\n" + color_code + "}\n"
+                + access.store_code,
+                absorb_code + access.store_code, margin)
             # without the colour staging the LDS holds only the planes; the
             # last step then stores each colour itself (a wave's 64 cells
             # of a row are contiguous), which pays where it raises the
@@ -214,11 +223,21 @@ class FusedProgram:
         return np.dtype(fields, align=True)
 
     def _render(self, defines, emit_body, absorb_body, absorb_plain,
-                margin, cdirect=False):
+                margin, cdirect=False, render=None):
         """The kernel source; with ``absorb_plain`` (the absorb body
         without the colour) the form that computes the colour only on a
-        launch's last step and never reads the stored colours."""
+        launch's last step and never reads the stored colours; with
+        ``render`` (the per-pixel render body and its extra kernel
+        arguments) the form whose tiles also write their viewport pixels
+        from the colours in LDS after the last step."""
         tx, ty = self.tile
+        render_args = render_epilogue = ""
+        if render is not None:
+            body, extra = render
+            render_args = (", const int *__restrict__ _rt, int *img, "
+                           "int zoom, int dx, int dy, int width" + extra)
+            render_epilogue = _RENDER_EPILOGUE.format(
+                w=self.width_prefix, render_body=body)
         params_struct = "".join(
             "%s p_%s;\n" % (_PARAM_FIELD[p.ctype][0], p.name)
             for p in self.params)
@@ -281,7 +300,8 @@ class FusedProgram:
             cfast=cfast, cdirect=int(cdirect), helpers=ranges.FDIV_HELPER,
             bvalue="%dULL" % (self.border_value & 0xFFFFFFFFFFFFFFFF),
             params_struct=params_struct, param_locals=param_locals,
-            zero_slots=zero_slots, tb_decl=tb_decl, step_loop=step_loop)
+            zero_slots=zero_slots, tb_decl=tb_decl, step_loop=step_loop,
+            render_args=render_args, render_epilogue=render_epilogue)
 
     @property
     def function(self):
@@ -297,6 +317,27 @@ class FusedProgram:
             self._module_fast = hip.Module.from_source(
                 self.source_fast, "bsca_fused_fast.hip")
         return self._module_fast.get_function("fused_bsca")
+
+    def render_function(self, model, fast):
+        """The form ``fast`` of the kernel with the render epilogue (its
+        arguments extended by the render tables, the image, zoom, pan,
+        viewport width and the model's render parameters), or None when
+        the model's render body is not the plain 2-D projection's."""
+        if fast not in self._forms:
+            return None
+        if fast not in self._render_fns:
+            function = None
+            body = _epilogue_body(model)
+            if body is not None:
+                extra = "".join(", %s %s" % (p.ctype, p.name)
+                                for p in model._render_params)
+                module = hip.Module.from_source(
+                    self._render(*self._forms[fast], render=(body, extra)),
+                    "bsca_fused_render.hip")
+                self._render_modules[fast] = module
+                function = module.get_function("fused_bsca")
+            self._render_fns[fast] = function
+        return self._render_fns[fast]
 
     def admits(self, bounds):
         """Whether stored colours within ``bounds`` (per-channel ``(min,
@@ -390,7 +431,7 @@ __device__ __forceinline__ int _wrap(int v, int n) {{
 extern "C" __global__ __launch_bounds__(256)
 void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all,
                 int3 *__restrict__ _col_all, const _Inst *__restrict__ _insts,
-                const int *__restrict__ _tile_inst, int _n_tiles, int _steps) {{
+                const int *__restrict__ _tile_inst, int _n_tiles, int _steps{render_args}) {{
     __shared__ _word _mp[2][_RA];
     {tb_decl}
 #if !_CDIRECT
@@ -516,9 +557,35 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
             }}
         }}
 #endif
-        __syncthreads();
+{render_epilogue}        __syncthreads();
     }}
 }}
+"""
+
+# The render epilogue: after the last step the colours of the output tile
+# are in LDS (_col), and the tile writes every viewport pixel whose cell it
+# holds with the model's render body.  The host's tables list, per cell
+# column (row), the pixel columns (rows) that show it -- _rt holds the
+# column offsets [w0 + 1], the row offsets [w1 + 1], the pixel columns
+# ordered by cell column [width] and the pixel rows ordered by cell row --
+# so a tile's pixels are one contiguous run of each list.  The render body
+# recomputes the cell of pixel i itself; a cell outside the tile (which the
+# tables exclude) would read black, never another tile's LDS.
+_RENDER_EPILOGUE = r"""        {{
+            const int *_rxo = _rt, *_ryo = _rt + _in.w0 + 1;
+            const int *_rxp = _ryo + _in.w1 + 1, *_ryp = _rxp + width;
+            const int _px0 = _rxo[_ox];
+            const int _npx = _rxo[min(_ox + _TX, _in.w0)] - _px0;
+            const int _py0 = _ryo[_oy];
+            const int _npy = _ryo[min(_oy + _TY, _in.w1)] - _py0;
+            const int {w}0 = _in.w0, {w}1 = _in.w1;
+            for (int _q = threadIdx.x; _q < _npx * _npy; _q += _NT) {{
+                const int _jy = _q / _npx, _jx = _q - _jy * _npx;
+                const unsigned i = (unsigned) _ryp[_py0 + _jy] * (unsigned) width
+                                   + (unsigned) _rxp[_px0 + _jx];
+{render_body}
+            }}
+        }}
 """
 
 
@@ -819,6 +886,54 @@ def _persistent_grid(n_tiles, function):
     return max(1, min(n_tiles, max(1, per_cu) * max(1, cus)))
 
 
+_COLOUR_LOAD = "int3 c = col[ii];\n"
+# the epilogue's colour of cell (x0, x1): the tile's LDS colours
+_COLOUR_LDS = (
+    "int3 c = ((unsigned) (x0 - _ox) < (unsigned) _TX\n"
+    "          && (unsigned) (x1 - _oy) < (unsigned) _TY)\n"
+    "    ? _col[(x1 - _oy) * _TX + (x0 - _ox)] : make_int3(0, 0, 0);\n")
+
+
+def _epilogue_body(model):
+    """The model's per-pixel render body reading the tile's LDS colours,
+    or None unless the renderer is the plain projection onto axes (0, 1)
+    of a 2-D field (``renderers.py:220-270``)."""
+    from xentica_amd.core.renderers import RendererPlain
+    renderer = model.renderer
+    if type(renderer) is not RendererPlain \
+            or tuple(renderer.projection_axes) != (0, 1) \
+            or model.topology.dimensions != 2:
+        return None
+    body = model.kernel_bodies.get('render', "")
+    if body.count(_COLOUR_LOAD) != 1:
+        return None
+    return body.replace(_COLOUR_LOAD, _COLOUR_LDS)
+
+
+def render_tables(size, width, height, zoom, dx, dy):
+    """
+    The render epilogue's pixel lists of a viewport: ``int32`` column
+    offsets ``[w0 + 1]``, row offsets ``[w1 + 1]``, pixel columns ordered
+    by the cell column they show ``[width]`` and pixel rows ordered by cell
+    row ``[height]``, concatenated.  A pixel shows the cell of the plain
+    renderer's body: ``(int) ((float) p / (float) zoom) + shift``, wrapped
+    onto the torus as ``renderers.py:220-270`` does.
+    """
+    parts_off, parts_pix = [], []
+    for count, shift, cells in ((width, dx, size[0]), (height, dy, size[1])):
+        pix = np.arange(count, dtype=np.int64)
+        cell = (pix.astype(np.float32) / np.float32(zoom)).astype(np.int64)
+        cell += shift
+        neg = cell < 0
+        cell[neg] = cells - (-cell[neg]) % cells
+        big = cell >= cells
+        cell[big] = cell[big] % cells
+        order = np.argsort(cell, kind="stable")
+        parts_off.append(np.searchsorted(cell[order], np.arange(cells + 1)))
+        parts_pix.append(pix[order])
+    return np.concatenate(parts_off + parts_pix).astype(np.int32)
+
+
 # the colour effect's store, and the last-step form's store straight to
 # HBM (output-tile cell _oi of the workgroup's tile, inside the field only)
 _COLOR_STORE = "col[i] = make_int3(new_r, new_g, new_b);"
@@ -909,6 +1024,9 @@ class FusedEngine:
         self._other = None
         self._in_other = False
         self._launches = {}
+        self._tables = (None, None)
+        #: frames rendered by a launch's epilogue (no separate render pass)
+        self.epilogue_frames = 0
         arrays = model.gpu.arrays
         arrays.cells.before_read = self.flush
         arrays.colors.before_read = self.flush
@@ -938,7 +1056,9 @@ class FusedEngine:
         if self._pending >= self.program.depth:
             self.flush()
 
-    def _run(self, steps, values):
+    def _run(self, steps, values, render=None):
+        """One launch of ``steps`` steps; ``render``: ``(fast, function,
+        extra arguments)`` of the render-epilogue form for this launch."""
         self._inst.refresh(values)
         arrays = self.model.gpu.arrays
         plane0 = arrays.cells.plane_ptr(self.model.cells_num)
@@ -949,7 +1069,16 @@ class FusedEngine:
         # launches prepared once per (direction, step count, arrays, form)
         key = (src, dst, colors, steps, fast)
         launch = self._launches.get(key)
-        if launch is None:
+        if render is not None:
+            fast, function, extra = render
+            grid = _persistent_grid(self._inst.n_tiles, function)
+            launch = function.prepare(
+                np.uint64(src), np.uint64(dst), np.uint64(colors),
+                np.uint64(self._inst.table_buf.ptr),
+                np.uint64(self._inst.tiles_buf.ptr),
+                np.int32(self._inst.n_tiles), np.int32(steps), *extra,
+                grid=(grid, 1, 1), block=(BLOCK, 1, 1))
+        elif launch is None:
             grid = _persistent_grid(self._inst.n_tiles, function)
             launch = function.prepare(
                 np.uint64(src), np.uint64(dst), np.uint64(colors),
@@ -978,6 +1107,47 @@ class FusedEngine:
                 self._other.ptr,
                 nbytes, None))
             self._in_other = False
+
+    def render_frame(self, render_params, pixels):
+        """
+        With steps queued, run them with the render epilogue: the last
+        launch's tiles also write the viewport's pixels from the colours
+        they hold in LDS, with the model's render body
+        (``renderers.py:220-270``), and no separate pass reads the colours
+        back.  Returns False (the caller renders from the arrays) when no
+        step is queued, the zoom is below 1 or the renderer is not the plain
+        2-D projection.
+        """
+        model = self.model
+        if not self._pending or model.zoom < 1:
+            return False
+        self._setup()
+        arrays = model.gpu.arrays
+        while self._pending > self.program.depth:
+            self._run(self.program.depth, self._pending_values)
+            self._pending -= self.program.depth
+        fast, _ = self.program.kernel_for(arrays.colors.bounds)
+        function = self.program.render_function(model, fast)
+        if function is None:
+            return False
+        args = model.renderer.get_args_vals(model)
+        zoom, dx, dy, width = (int(v) for v in args[2:6])
+        view = (zoom, dx, dy, width, pixels // width)
+        if self._tables[0] != view:
+            table = render_tables(model.size, width, pixels // width,
+                                  zoom, dx, dy)
+            buf = hip.Buffer(table.nbytes)
+            buf.upload(table)
+            self._tables = (view, buf)
+        extra = [np.uint64(self._tables[1].ptr),
+                 np.uint64(arrays.img.device_ptr()), np.int32(zoom),
+                 np.int32(dx), np.int32(dy), np.int32(width)]
+        extra += list(render_params)
+        self._run(self._pending, self._pending_values,
+                  render=(fast, function, extra))
+        self._pending = 0
+        self.epilogue_frames += 1
+        return True
 
     def load_state(self, cells, colors):
         """Replace the state (queued steps are applied first)."""
