@@ -22,7 +22,11 @@ The JSON line also carries:
   time, ``valu_frac`` the VALU issue fraction from the committed counters;
 * ``cpu_baseline`` -- the same model's generated emit/absorb code compiled
   for the host with OpenMP (``runtime/hostc.py``) on a bounded sample;
-* ``plain_engine`` -- the reference-structured GPU path on the same field.
+* ``plain_engine`` -- the reference-structured GPU path on the same field;
+* ``steady_state`` (when ``--steps`` is below 2000) -- 2000 more steps of
+  the same field timed the same way: a short run pays each launch's fixed
+  cost (about 11 us) and the first launch's wait for ``depth`` queued
+  steps on few launches.
 
 Usage: ``python bench.py [--gpus 1] [--steps 2000] [--warmup 200]``.
 """
@@ -79,6 +83,28 @@ def headline(steps, warmup):
     stop.synchronize()
     hip.device().synchronize()
     return model, start.elapsed_ms(stop)
+
+
+def steady_state(model, steps=2000):
+    """``steps`` more calls of ``model.step()`` (continuing the headline
+    field) between HIP events: the rate once the fixed costs of a short
+    run (the first launch waiting for ``depth`` queued steps, a shorter
+    last launch) are spread over many launches."""
+    from xentica_amd.runtime import hip
+    engine = model._engine
+    engine.finish()
+    hip.device().synchronize()
+    start, stop = hip.Event(), hip.Event()
+    start.record(None)
+    for _ in range(steps):
+        model.step()
+    engine.finish()
+    stop.record(None)
+    stop.synchronize()
+    ms = start.elapsed_ms(stop)
+    return {"steps": steps, "ms_per_step": ms / steps,
+            "value": model.cells_num * steps / (ms / 1e3),
+            "unit": "cell-updates/s"}
 
 
 def counters():
@@ -909,6 +935,7 @@ def main():
     verified = crc == plain_crc
     log("verification against the plain engine after %d steps: %s"
         % (total, verified))
+    steady = steady_state(model) if args.steps < 2000 else None
     achieved = value * BYTES_PER_CELL_UPDATE / 1e9
     prof = counters()
     traffic = prof.get("hbm_bytes_per_launch")
@@ -953,6 +980,8 @@ def main():
                                 "HBM round trip)",
         },
     }
+    if steady:
+        result["steady_state"] = steady
     if plain_rate:
         result["plain_engine"] = plain_rate
         result["speedup_vs_plain"] = value / plain_rate["value"]

@@ -15,7 +15,8 @@ import pytest
 
 from oracle import oracle as O
 from tests.emulation import life_batch as E
-from xentica_amd.runtime.bits_batch import plan_lanes, layout, pitch_of
+from xentica_amd.runtime.bits_batch import (
+    plan_lanes, layout, pitch_of, balanced_strips)
 
 
 def pack_rows(cells, width, height):
@@ -101,7 +102,7 @@ def test_plan_covers_the_batch_workload():
     from examples.workloads import batch_experiments
     sizes = [exp.size for exp in batch_experiments()]
     bases, total = layout(sizes)
-    for strip_rows in (16, 48):
+    for strip_rows in (16, 39, 48):
         lanes, wave_rows = plan_lanes(sizes, strip_rows)
         count = np.zeros(total, dtype=np.int64)
         active = lanes[:, :, 3] > 0
@@ -113,3 +114,18 @@ def test_plan_covers_the_batch_workload():
             count[start:start + rows * pitch_of(width):pitch_of(width)] += 1
         assert np.all(count == 1)
         assert np.all(wave_rows >= lanes[:, :, 3].max(axis=1))
+
+
+def test_balanced_strips_fill_whole_waves_per_simd():
+    """The heights the autotuner adds: the lowest whose plan has at most
+    two (three) waves per SIMD of a 1024-SIMD device, so one row less
+    would exceed that count."""
+    from examples.workloads import batch_experiments
+    sizes = [exp.size for exp in batch_experiments()]
+    heights = balanced_strips(sizes, 1024)
+    assert len(heights) == 2
+    for height, per_simd in zip(heights, (3, 2)):
+        assert len(plan_lanes(sizes, height)[1]) <= per_simd * 1024
+        assert len(plan_lanes(sizes, height - 1)[1]) > per_simd * 1024
+    # a batch too small to reach the counts gets no height
+    assert balanced_strips([(64, 64)], 1024) == []

@@ -24,7 +24,8 @@ from xentica_amd.core.exceptions import XenticaException
 from xentica_amd.runtime import hip
 from xentica_amd.runtime.hostc import LifeLikeProbe
 
-__all__ = ['plan_lanes', 'BitsBatch', 'autotune', 'MAX_TILE_WORDS', 'WAVE']
+__all__ = ['plan_lanes', 'balanced_strips', 'BitsBatch', 'autotune',
+           'MAX_TILE_WORDS', 'WAVE']
 
 #: Lanes of a wavefront, and stored words of one column tile (two lanes of
 #: a wave's 64 are the tile's halo).
@@ -73,16 +74,9 @@ def layout(sizes):
     return bases, total
 
 
-def plan_lanes(sizes, strip_rows=48):
-    """
-    The lane descriptors of one launch over instances of ``sizes``.
-
-    Returns ``(lanes, wave_rows)``: ``lanes`` is int32 ``[waves, 64, 4]``
-    (instance or -1, extended word column, first output row, output rows
-    or 0 for lanes that store nothing) and ``wave_rows`` the most output
-    rows of each wave.  Every stored word of every instance is the output
-    of exactly one lane.
-    """
+def _pack(sizes, strip_rows):
+    """The waves of a plan: ``[free lanes, rows, [items]]`` each, an item
+    being ``(rows, lanes, instance, first word, end word, first row)``."""
     items = []
     for inst, (width, height) in enumerate(sizes):
         width, height = int(width), int(height)
@@ -120,6 +114,46 @@ def plan_lanes(sizes, strip_rows=48):
         wave[2].append(item)
         if wave[0] < 3:
             open_waves.remove(wave)
+    return waves
+
+
+def balanced_strips(sizes, simds, per_simd=(2, 3), lowest=8, highest=512):
+    """
+    Strip heights whose plans come closest to, without exceeding, a whole
+    number of waves per SIMD: for each count in ``per_simd``, the lowest
+    height in ``[lowest, highest]`` whose plan has at most ``count * simds``
+    waves (none when even the lowest height stays under that count).  A VALU-bound launch whose waves are one short of filling the
+    SIMDs evenly runs as long as one that fills them, so these heights are
+    where the halo rows are fewest for a given finishing time.
+    """
+    out = []
+    for count in per_simd:
+        limit = count * simds
+        low, high = lowest, highest
+        if len(_pack(sizes, high)) > limit or \
+                len(_pack(sizes, low)) <= limit:
+            continue  # no height in range reaches (or stays under) it
+        while low < high:
+            mid = (low + high) // 2
+            if len(_pack(sizes, mid)) <= limit:
+                high = mid
+            else:
+                low = mid + 1
+        out.append(low)
+    return sorted(set(out))
+
+
+def plan_lanes(sizes, strip_rows=48):
+    """
+    The lane descriptors of one launch over instances of ``sizes``.
+
+    Returns ``(lanes, wave_rows)``: ``lanes`` is int32 ``[waves, 64, 4]``
+    (instance or -1, extended word column, first output row, output rows
+    or 0 for lanes that store nothing) and ``wave_rows`` the most output
+    rows of each wave.  Every stored word of every instance is the output
+    of exactly one lane.
+    """
+    waves = _pack(sizes, strip_rows)
     lanes = np.zeros((len(waves), WAVE, 4), dtype=np.int32)
     lanes[:, :, 0] = -1
     wave_rows = np.zeros(len(waves), dtype=np.int32)
@@ -348,22 +382,30 @@ class BitsBatch:
         return float((self.lanes[:, :, 3] > 0).sum()) / self.lanes[:, :, 0].size
 
 
-#: Temporal depths and strip heights the autotuner tries.
+#: Temporal depths and fixed strip heights the autotuner tries (beside
+#: the heights :func:`balanced_strips` finds for the batch).
 TUNE_DEPTHS = (4, 6, 8)
-TUNE_STRIPS = (16, 24, 32, 48)
+TUNE_STRIPS = (32, 48)
 
 
 def autotune(models, trial_steps=48, depths=TUNE_DEPTHS,
-             strip_rows=TUNE_STRIPS, stream=None, log=None):
+             strip_rows=None, stream=None, log=None):
     """
     Pick the temporal depth and the strip height of a bit-sliced batch by
     measurement: every pair runs once to warm up, then ``trial_steps``
     steps (whole launches, graph-captured) between HIP events, from the
-    models' states (which are not advanced).  Returns ``(batch, timings)``
-    with the fastest :class:`BitsBatch` ready to run from the models'
-    states and ``{(depth, strip_rows): cell-updates per second}``.
+    models' states (which are not advanced).  The heights tried are
+    ``strip_rows``, or by default :data:`TUNE_STRIPS` and the heights that
+    bring the plan just under two and three waves per SIMD of this device.
+    Returns ``(batch, timings)`` with the fastest :class:`BitsBatch` ready
+    to run from the models' states and ``{(depth, strip_rows): cell-updates
+    per second}``.
     """
     models = list(models)
+    if strip_rows is None:
+        simds = 4 * hip.device().compute_units
+        strip_rows = sorted(set(TUNE_STRIPS) | set(balanced_strips(
+            [model.size for model in models], simds)))
     stream = stream or hip.Stream()
     cells = sum(model.cells_num for model in models)
     timings, best = {}, None
