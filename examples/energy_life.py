@@ -132,19 +132,23 @@ class EnergyLifeSmall(EnergyLifeExperiment):
     size = (96, 80)
 
 
+def field_position(model, name):
+    """``(shift, width)`` of property ``name`` in the main container (the
+    fields in declaration order from bit 0)."""
+    shift = 0
+    for prop_name, prop in model.main.properties.items():
+        if prop_name == name:
+            return shift, prop.bit_width
+        shift += prop.bit_width
+    raise KeyError(name)
+
+
 def field_values(model, name):
     """The values of property ``name`` of every cell of ``model``'s main
     plane (the container's bit layout: fields in declaration order from
     bit 0; 128-bit words as two little-endian uint64 halves)."""
     import numpy as np
-    shift = 0
-    for prop_name, prop in model.main.properties.items():
-        if prop_name == name:
-            width = prop.bit_width
-            break
-        shift += prop.bit_width
-    else:
-        raise KeyError(name)
+    shift, width = field_position(model, name)
     words = model.gpu.arrays.cells.get()[:model.cells_num]
     lo = words['lo'].astype(np.uint64)
     hi = words['hi'].astype(np.uint64)

@@ -122,6 +122,17 @@ int xen_crc32_u8_partials(const uint8_t *data, long long n, long long chunk,
 int xen_crc32_bits_partials(const uint32_t *bits, long long n_cells, long long chunk,
                             uint32_t *partials, xen_stream stream);
 
+/* ---- sum of one field over a plane of container words ----
+ * No reference counterpart: the reference checks conserved quantities on
+ * the host after `cells.get()` (examples/conserved_life.py:237-256).
+ * partials[b] = sum over the cells block b visits (grid-stride, n_partials
+ * blocks of 256 threads) of (word >> shift) & (2^width - 1); the host adds
+ * the n_partials values.  word_bytes is 1, 2, 4, 8 or 16 (two little-endian
+ * 64-bit halves), width 1..32, shift + width within the word; words aligned
+ * to their size (16-byte words to 16). */
+int xen_field_sum_partials(const void *words, long long n, int word_bytes, int shift, int width,
+                           uint64_t *partials, int n_partials, xen_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

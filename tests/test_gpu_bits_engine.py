@@ -195,6 +195,24 @@ def test_headline_through_model_step():
         assert model._engine.crc32() == want, "after %d" % (250 * block + 250)
 
 
+def test_headline_every_step_validation_run():
+    """The headline workload's validation run: Life 16384^2 through
+    ``GameOfLife.step()`` on the bits engine and on the plain engine, the
+    CRC of the canonical uint8 plane compared after every one of 2000
+    steps (each read launches the one step queued; the timed run's 8-step
+    launches are checked by the test above)."""
+    model = GameOfLife(Life16kExperiment)
+    assert isinstance(model._engine, BitsEngine)
+    plain = GameOfLife(with_engine(Life16kExperiment, "plain"))
+    print("Life 16384^2, CRC after each of 2000 steps", flush=True)
+    for step in range(2000):
+        model.step()
+        plain.step()
+        want = kernels.crc32_u8(plain.gpu.arrays.cells.device_ptr(),
+                                plain.cells_num)
+        assert model._engine.crc32() == want, "step %d" % (step + 1)
+
+
 @pytest.mark.parametrize("zoom,pos,viewport", [(1, [0, 0], (640, 360)),
                                                (3, [17, 5], (200, 120)),
                                                (2, [630, 350], (333, 77))])

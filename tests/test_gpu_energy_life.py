@@ -3,7 +3,8 @@ EnergyLife on the GPU engines against the oracle.
 
 Per-step CRC-32 of the 128-bit main plane and the colours on a 96x80
 torus (plain and fused engines); the 4096x4096 workload a few steps
-against the oracle; and conserved total energy over a longer run.
+against the oracle; conserved total energy over a longer run, and after
+every one of the workload's 5000 steps (summed on the device).
 """
 import binascii
 
@@ -12,7 +13,9 @@ import pytest
 
 from oracle import oracle
 from examples.energy_life import (
-    EnergyLife, EnergyLifeExperiment, EnergyLifeSmall)
+    EnergyLife, EnergyLifeExperiment, EnergyLifeSmall, field_position,
+    field_values)
+from xentica_amd.runtime import kernels
 
 pytestmark = pytest.mark.gpu
 
@@ -66,6 +69,48 @@ def test_full_size_energy_conserved():
         assert energy.sum() == total
         assert energy.max() <= 255
         assert 0.05 < mode.mean() < 0.95
+
+
+def device_energy(model):
+    """Total energy of the model's current main plane, summed on the
+    device."""
+    shift, width = field_position(model, "energy")
+    model._engine.flush()
+    cells = model.gpu.arrays.cells
+    return kernels.field_sum(cells.plane_ptr(model.cells_num),
+                             model.cells_num, cells.dtype.itemsize, shift,
+                             width)
+
+
+def test_device_field_sum_equals_host():
+    """The device sum of each field equals NumPy's over the downloaded
+    plane, before and after some steps."""
+    model = EnergyLife(EnergyLifeExperiment)
+    for steps in (0, 7):
+        for _ in range(steps):
+            model.step()
+        model._engine.flush()
+        cells = model.gpu.arrays.cells
+        for name in ("energy", "genome", "mode", "rng"):
+            shift, width = field_position(model, name)
+            want = int(field_values(model, name).sum())
+            got = kernels.field_sum(cells.plane_ptr(model.cells_num),
+                                    model.cells_num, cells.dtype.itemsize,
+                                    shift, width)
+            assert got == want, (steps, name)
+
+
+def test_energy_conserved_every_step_of_the_workload():
+    """The C3 validation run: 5000 steps of EnergyLife 4096x4096 (one
+    step per launch), the total energy after every step equal to the
+    seed's."""
+    model = EnergyLife(EnergyLifeExperiment)
+    total = device_energy(model)
+    assert total == int(field_values(model, "energy").sum())
+    print("EnergyLife 4096^2, 5000 steps, energy %d" % total, flush=True)
+    for step in range(5000):
+        model.step()
+        assert device_energy(model) == total, "after step %d" % (step + 1)
 
 
 def test_batch_of_uneven_fields():

@@ -101,6 +101,52 @@ __global__ __launch_bounds__(kBlock) void crc_bits_kernel(const uint32_t *__rest
     partials[c] = crc;
 }
 
+// The sum of one bit field of every container word of a plane: field value
+// (word >> shift) & (2^width - 1), width <= 32, summed in 64 bits per thread
+// over a grid-stride range, then across the block (wave shuffles, then LDS)
+// into partials[blockIdx.x].  Words of 16 bytes are two little-endian
+// 64-bit halves.
+template <typename W>
+__device__ __forceinline__ uint64_t field_of(const W *words, long long i, int shift,
+                                             uint32_t mask) {
+    return (static_cast<uint64_t>(words[i]) >> shift) & mask;
+}
+
+template <>
+__device__ __forceinline__ uint64_t field_of<ulonglong2>(const ulonglong2 *words, long long i,
+                                                         int shift, uint32_t mask) {
+    const ulonglong2 w = words[i];
+    uint64_t v;
+    if (shift >= 64)
+        v = w.y >> (shift - 64);
+    else if (shift == 0)
+        v = w.x;
+    else
+        v = (w.x >> shift) | (w.y << (64 - shift));
+    return v & mask;
+}
+
+template <typename W>
+__global__ __launch_bounds__(kBlock) void field_sum_kernel(const W *__restrict__ words,
+                                                           long long n, int shift,
+                                                           uint32_t mask,
+                                                           uint64_t *__restrict__ partials) {
+    __shared__ uint64_t wave_sums[kBlock / 64];
+    uint64_t sum = 0;
+    const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+    for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += stride)
+        sum += field_of(words, i, shift, mask);
+    for (int offset = 32; offset > 0; offset >>= 1) sum += __shfl_down(sum, offset, 64);
+    if ((threadIdx.x & 63) == 0) wave_sums[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t total = 0;
+        for (int w = 0; w < kBlock / 64; ++w) total += wave_sums[w];
+        partials[blockIdx.x] = total;
+    }
+}
+
 // The low byte of every int32 value (what C's and NumPy's int32 -> uint8
 // casts give), four values per thread: one 16-byte load, one 4-byte store.
 __global__ __launch_bounds__(kBlock) void low_bytes_kernel(const int32_t *__restrict__ src,
@@ -221,6 +267,44 @@ int xen_crc32_bits_partials(const uint32_t *bits, long long n_cells, long long c
                        reinterpret_cast<hipStream_t>(stream), bits, n_cells, chunk, n_chunks,
                        partials);
     return xen::check(hipGetLastError(), "crc_bits_kernel launch");
+}
+
+int xen_field_sum_partials(const void *words, long long n, int word_bytes, int shift, int width,
+                           uint64_t *partials, int n_partials, xen_stream stream) {
+    if (n < 0 || n_partials < 1)
+        return xen::fail("xen_field_sum_partials: negative count or no partials", 22);
+    if (width < 1 || width > 32 || shift < 0 || shift + width > 8 * word_bytes)
+        return xen::fail("xen_field_sum_partials: field outside the word (width 1..32)", 22);
+    if (reinterpret_cast<uintptr_t>(words) % (word_bytes < 16 ? word_bytes : 16))
+        return xen::fail("xen_field_sum_partials: words not aligned to their size", 22);
+    const uint32_t mask = width == 32 ? 0xFFFFFFFFu : (1u << width) - 1u;
+    const dim3 grid(n_partials), block(kBlock);
+    const auto st = reinterpret_cast<hipStream_t>(stream);
+    switch (word_bytes) {
+    case 1:
+        hipLaunchKernelGGL(field_sum_kernel<uint8_t>, grid, block, 0, st,
+                           static_cast<const uint8_t *>(words), n, shift, mask, partials);
+        break;
+    case 2:
+        hipLaunchKernelGGL(field_sum_kernel<uint16_t>, grid, block, 0, st,
+                           static_cast<const uint16_t *>(words), n, shift, mask, partials);
+        break;
+    case 4:
+        hipLaunchKernelGGL(field_sum_kernel<uint32_t>, grid, block, 0, st,
+                           static_cast<const uint32_t *>(words), n, shift, mask, partials);
+        break;
+    case 8:
+        hipLaunchKernelGGL(field_sum_kernel<uint64_t>, grid, block, 0, st,
+                           static_cast<const uint64_t *>(words), n, shift, mask, partials);
+        break;
+    case 16:
+        hipLaunchKernelGGL(field_sum_kernel<ulonglong2>, grid, block, 0, st,
+                           static_cast<const ulonglong2 *>(words), n, shift, mask, partials);
+        break;
+    default:
+        return xen::fail("xen_field_sum_partials: word_bytes must be 1, 2, 4, 8 or 16", 22);
+    }
+    return xen::check(hipGetLastError(), "field_sum_kernel launch");
 }
 
 }  // extern "C"

@@ -15,6 +15,7 @@ __all__ = [
     'life_bits_launch', 'life_bits_run', 'life3d_bits_step',
     'life3d_u8_direct_step', 'bits_pack', 'bits_unpack',
     'crc32_u8', 'crc32_bits', 'CRC_CHUNK', 'low_bytes', 'bits_expand',
+    'field_sum',
 ]
 
 _vp = ctypes.c_void_p
@@ -34,6 +35,7 @@ _SIGNATURES = {
     'xen_bits_expand': [_vp, _vp, _vp, _ll, _vp, _vp, _vp],
     'xen_crc32_u8_partials': [_vp, _ll, _ll, _vp, _vp],
     'xen_crc32_bits_partials': [_vp, _ll, _ll, _vp, _vp],
+    'xen_field_sum_partials': [_vp, _ll, _i, _i, _i, _vp, _i, _vp],
 }
 _BOUND = False
 
@@ -141,3 +143,23 @@ def crc32_bits(bits_ptr, n_cells, stream=None, chunk=CRC_CHUNK):
     host = np.empty(n_chunks, dtype=np.uint32)
     parts.download(host, stream)
     return combine_partials(host, chunk, n_cells)
+
+
+#: Blocks (partial sums) of :func:`field_sum`.
+FIELD_SUM_BLOCKS = 1024
+_FIELD_PARTIALS = []
+
+
+def field_sum(words_ptr, n, word_bytes, shift, width, stream=None):
+    """Sum over ``n`` container words at ``words_ptr`` of the field
+    ``(word >> shift) & (2**width - 1)``, computed on the device (an exact
+    integer)."""
+    if not _FIELD_PARTIALS:
+        _FIELD_PARTIALS.append(hip.Buffer(8 * FIELD_SUM_BLOCKS))
+    parts = _FIELD_PARTIALS[0]
+    hip.check(_lib().xen_field_sum_partials(
+        words_ptr, n, word_bytes, shift, width, parts.ptr, FIELD_SUM_BLOCKS,
+        _s(stream)), "xen_field_sum_partials")
+    host = np.empty(FIELD_SUM_BLOCKS, dtype=np.uint64)
+    parts.download(host, stream)
+    return int(host.sum(dtype=np.uint64))
