@@ -18,8 +18,9 @@ The JSON line also carries:
 * ``roofline`` -- algorithmic bytes (one bit read + one bit written per
   cell-update = 0.25 B) over the timed region against MI355X's 8 TB/s HBM
   peak; ``traffic`` is the HBM bytes of one 8-step launch from the
-  committed rocprofv3 counters, ``hbm_frac`` that traffic over the launch
-  time, ``valu_frac`` the VALU issue fraction from the committed counters;
+  committed rocprofv3 counters, ``hbm_frac`` that traffic over the timed
+  region's average launch time, ``valu_frac`` the committed count of VALU
+  instructions per launch over the same time at the VALU issue peak;
 * ``cpu_baseline`` -- the same model's generated emit/absorb code compiled
   for the host with OpenMP (``runtime/hostc.py``) on a bounded sample;
 * ``plain_engine`` -- the reference-structured GPU path on the same field;
@@ -46,6 +47,8 @@ METRIC = ("cell-updates/s and % HBM roofline; batched and surrogate speedup "
           "vs plain")
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 0.25  # 1 bit read + 1 bit written
+# wave64 VALU instructions per second: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 SIZE = (16384, 16384)
 WORD = 42
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "life_bits_16384_traffic.json")
@@ -975,7 +978,9 @@ def main():
             "traffic": traffic,
             "hbm_frac": (traffic / launch_s / 1e9 / HBM_PEAK_GBS
                          if traffic else None),
-            "valu_frac": prof.get("valu_frac"),
+            "valu_frac": (prof["sq_insts_valu_per_launch"] / launch_s
+                          / VALU_ISSUE_PEAK
+                          if prof.get("sq_insts_valu_per_launch") else None),
             "binding_resource": "VALU issue (temporal blocking: 8 steps per "
                                 "HBM round trip)",
         },

@@ -36,6 +36,9 @@ constexpr int kWavesPerGroup = 4;
 constexpr int kRsrcFlags = 0x00020000;
 // keep several waves per SIMD resident (caps VGPRs at 512 / this)
 constexpr int kMinWavesPerSimd = 4;
+// s_waitcnt immediate of gfx9-family parts: vmcnt 0, expcnt and lgkmcnt at
+// their maxima (not waited on)
+constexpr int kVmcntZero = 0x0F70;
 
 __device__ __forceinline__ int wrap(int v, int n) {
     v %= n;
@@ -135,46 +138,75 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
 
     Rule<kFast> rule{born, sustain};
     Window<K> w;
+    // The steady state feeds nine rows per iteration from the registers
+    // `cur`; at the top of each iteration the nine rows of the next one are
+    // requested into `nxt`, and `cur = nxt` moves them over at the end.  The
+    // compiler turns that hand-over (and, in practice, any other order of
+    // loads across the back edge) into register copies at the loop head,
+    // and a copy waits for its load: here those loads were issued a whole
+    // iteration (72 row-levels of work) earlier, so the wait is normally
+    // already satisfied and the stores stay in flight.  The prologue's 2K
+    // rows and the first nine are all requested before the first feed.
+    // Rows past L-1 are wrapped rows: in bounds, loaded and never fed.
+    constexpr int P0 = (2 * K) % 3, P1 = (2 * K + 1) % 3, P2 = (2 * K + 2) % 3;
+    uint32_t pro[2 * K];
+#pragma unroll
+    for (int r = 0; r < 2 * K; ++r) pro[r] = load_row(r);
+    int i = 2 * K;
+    uint32_t cur[9], nxt[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) cur[r] = load_row(i + r);
     // prologue: rows 0 .. 2K-1 fill the pipeline (no output yet)
 #pragma unroll
-    for (int i = 0; i < 2 * K; ++i) {
-        const uint32_t in = load_row(i);
-        switch (i % 3) {
-            case 0: feed<K, kFast, 0, false>(w, in, i, rule); break;
-            case 1: feed<K, kFast, 1, false>(w, in, i, rule); break;
-            default: feed<K, kFast, 2, false>(w, in, i, rule); break;
+    for (int r = 0; r < 2 * K; ++r) {
+        switch (r % 3) {
+            case 0: feed<K, kFast, 0, false>(w, pro[r], r, rule); break;
+            case 1: feed<K, kFast, 1, false>(w, pro[r], r, rule); break;
+            default: feed<K, kFast, 2, false>(w, pro[r], r, rule); break;
         }
     }
-    // steady state: rows 2K .. L-1, three per iteration so that the
-    // window slots are compile-time constants; input rows are loaded two
-    // iterations (six feeds) ahead of their use, which covers the load
-    // latency with fewer waves per SIMD (rows past L-1 are wrapped rows,
-    // in bounds and unused)
-    constexpr int P0 = (2 * K) % 3, P1 = (2 * K + 1) % 3, P2 = (2 * K + 2) % 3;
-    int i = 2 * K;
-    uint32_t a = load_row(i), b = load_row(i + 1), c = load_row(i + 2);
-    uint32_t na = load_row(i + 3), nb = load_row(i + 4), nc = load_row(i + 5);
-    for (; i + 2 < L; i += 3) {
-        const uint32_t fa = load_row(i + 6), fb = load_row(i + 7), fc = load_row(i + 8);
-        feed<K, kFast, P0, true>(w, a, i, rule);
-        store_row(level_k_row<K, P0>(w), i - 2 * K);
-        feed<K, kFast, P1, true>(w, b, i + 1, rule);
-        store_row(level_k_row<K, P1>(w), i + 1 - 2 * K);
-        feed<K, kFast, P2, true>(w, c, i + 2, rule);
-        store_row(level_k_row<K, P2>(w), i + 2 - 2 * K);
-        a = na;
-        b = nb;
-        c = nc;
-        na = fa;
-        nb = fb;
-        nc = fc;
+    // rows row .. row+2 (slots P0, P1, P2 by row mod 3), each output row
+    // r = row - 2K stored as its level-K row leaves the pipeline
+    auto feed3 = [&](uint32_t x0, uint32_t x1, uint32_t x2, int row) {
+        feed<K, kFast, P0, true>(w, x0, row, rule);
+        store_row(level_k_row<K, P0>(w), row - 2 * K);
+        feed<K, kFast, P1, true>(w, x1, row + 1, rule);
+        store_row(level_k_row<K, P1>(w), row + 1 - 2 * K);
+        feed<K, kFast, P2, true>(w, x2, row + 2, rule);
+        store_row(level_k_row<K, P2>(w), row + 2 - 2 * K);
+    };
+    // Every load so far has been consumed or is long complete: wait for them
+    // here, so that the loop head's waits are counted from the previous
+    // iteration alone (the compiler merges the entry path into the loop
+    // head's count, and the prologue's load order would otherwise make each
+    // hand-over wait for the previous iteration's stores as well).
+    __builtin_amdgcn_s_waitcnt(kVmcntZero);
+    for (; i + 8 < L; i += 9) {
+#pragma unroll
+        for (int r = 0; r < 9; ++r) nxt[r] = load_row(i + 9 + r);
+        feed3(cur[0], cur[1], cur[2], i);
+        feed3(cur[3], cur[4], cur[5], i + 3);
+        feed3(cur[6], cur[7], cur[8], i + 6);
+#pragma unroll
+        for (int r = 0; r < 9; ++r) cur[r] = nxt[r];
     }
-    if (i < L) {
-        feed<K, kFast, P0, true>(w, a, i, rule);
-        store_row(level_k_row<K, P0>(w), i - 2 * K);
-        if (i + 1 < L) {
-            feed<K, kFast, P1, true>(w, b, i + 1, rule);
-            store_row(level_k_row<K, P1>(w), i + 1 - 2 * K);
+    // the last 0 .. 8 rows
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        if (i + r >= L) break;
+        switch ((2 * K + r) % 3) {
+            case 0:
+                feed<K, kFast, 0, true>(w, cur[r], i + r, rule);
+                store_row(level_k_row<K, 0>(w), i + r - 2 * K);
+                break;
+            case 1:
+                feed<K, kFast, 1, true>(w, cur[r], i + r, rule);
+                store_row(level_k_row<K, 1>(w), i + r - 2 * K);
+                break;
+            default:
+                feed<K, kFast, 2, true>(w, cur[r], i + r, rule);
+                store_row(level_k_row<K, 2>(w), i + r - 2 * K);
+                break;
         }
     }
 }
