@@ -208,27 +208,53 @@ void life_batch_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ 
     }
     constexpr int P0 = (2 * K) % 3, P1 = (2 * K + 1) % 3, P2 = (2 * K + 2) % 3;
     int i = 2 * K;
-    // one iteration (three feeds) of loads in flight ahead of their use;
-    // rows past L - 1 are wrapped rows, in bounds and unused
-    RowLoad a = load_next(), b = load_next(), c = load_next();
-    for (; i + 2 < L; i += 3) {
-        const RowLoad na = load_next(), nb = load_next(), nc = load_next();
-        feed_rows<K, P0, true>(w, word_of(a), i, rule);
-        store_row(top_row<K, P0>(w), i - 2 * K);
-        feed_rows<K, P1, true>(w, word_of(b), i + 1, rule);
-        store_row(top_row<K, P1>(w), i + 1 - 2 * K);
-        feed_rows<K, P2, true>(w, word_of(c), i + 2, rule);
-        store_row(top_row<K, P2>(w), i + 2 - 2 * K);
-        a = na;
-        b = nb;
-        c = nc;
+    // Six rows per iteration from `cur`; the next six are requested into
+    // `nxt` at the top of the iteration and handed over at its end.  The
+    // compiler makes that hand-over register copies at the loop head, and a
+    // copy waits for its load: these loads were issued a whole iteration
+    // earlier, so the wait leaves the stores in flight (on gfx9 `vmcnt`
+    // counts stores as well).  Rows past L - 1 are wrapped rows, in bounds
+    // and unused.
+    RowLoad cur[6], nxt[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) cur[r] = load_next();
+    // Every earlier load is consumed or long complete: wait for them here,
+    // so that the loop head's waits are counted from the previous iteration
+    // alone (the compiler merges the entry path into the loop head's count).
+    __builtin_amdgcn_s_waitcnt(kVmcntZero);
+    auto feed3 = [&](const RowLoad &x0, const RowLoad &x1, const RowLoad &x2, int row) {
+        feed_rows<K, P0, true>(w, word_of(x0), row, rule);
+        store_row(top_row<K, P0>(w), row - 2 * K);
+        feed_rows<K, P1, true>(w, word_of(x1), row + 1, rule);
+        store_row(top_row<K, P1>(w), row + 1 - 2 * K);
+        feed_rows<K, P2, true>(w, word_of(x2), row + 2, rule);
+        store_row(top_row<K, P2>(w), row + 2 - 2 * K);
+    };
+    for (; i + 5 < L; i += 6) {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) nxt[r] = load_next();
+        feed3(cur[0], cur[1], cur[2], i);
+        feed3(cur[3], cur[4], cur[5], i + 3);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) cur[r] = nxt[r];
     }
-    if (i < L) {
-        feed_rows<K, P0, true>(w, word_of(a), i, rule);
-        store_row(top_row<K, P0>(w), i - 2 * K);
-        if (i + 1 < L) {
-            feed_rows<K, P1, true>(w, word_of(b), i + 1, rule);
-            store_row(top_row<K, P1>(w), i + 1 - 2 * K);
+    // the last 0 .. 5 rows
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        if (i + r >= L) break;
+        switch ((2 * K + r) % 3) {
+            case 0:
+                feed_rows<K, 0, true>(w, word_of(cur[r]), i + r, rule);
+                store_row(top_row<K, 0>(w), i + r - 2 * K);
+                break;
+            case 1:
+                feed_rows<K, 1, true>(w, word_of(cur[r]), i + r, rule);
+                store_row(top_row<K, 1>(w), i + r - 2 * K);
+                break;
+            default:
+                feed_rows<K, 2, true>(w, word_of(cur[r]), i + r, rule);
+                store_row(top_row<K, 2>(w), i + r - 2 * K);
+                break;
         }
     }
 }

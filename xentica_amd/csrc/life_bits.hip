@@ -36,9 +36,6 @@ constexpr int kWavesPerGroup = 4;
 constexpr int kRsrcFlags = 0x00020000;
 // keep several waves per SIMD resident (caps VGPRs at 512 / this)
 constexpr int kMinWavesPerSimd = 4;
-// s_waitcnt immediate of gfx9-family parts: vmcnt 0, expcnt and lgkmcnt at
-// their maxima (not waited on)
-constexpr int kVmcntZero = 0x0F70;
 
 __device__ __forceinline__ int wrap(int v, int n) {
     v %= n;

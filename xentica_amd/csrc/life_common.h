@@ -32,6 +32,9 @@ template <uint8_t kTable>
 __device__ __forceinline__ uint32_t bitop3(uint32_t a, uint32_t b, uint32_t c) {
     return __builtin_amdgcn_bitop3_b32(a, b, c, kTable);
 }
+// s_waitcnt immediate of gfx9-family parts: vmcnt 0, expcnt and lgkmcnt at
+// their maxima (not waited on)
+constexpr int kVmcntZero = 0x0F70;
 constexpr uint8_t kXor3 = bitop3_table([](bool a, bool b, bool c) { return a ^ b ^ c; });
 constexpr uint8_t kMaj3 = bitop3_table([](bool a, bool b, bool c) { return (a + b + c) >= 2; });
 constexpr uint8_t kMux = bitop3_table([](bool a, bool b, bool c) { return a ? b : c; });
