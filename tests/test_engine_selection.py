@@ -22,7 +22,7 @@ class _Ragged(GOLExperiment):
     (GameOfLife, GOLExperiment, "bits"),
     (LifelikeCA, DiamoebaExperiment, "bits"),
     (LifeLike3D, _Small3D, "bits"),
-    (GameOfLife, _Ragged, "fused"),
+    (GameOfLife, _Ragged, "bits"),
     (GameOfLifeStatic, GOLExperiment, "fused"),
     (GameOfLifeColor, GOLExperimentColor, "fused"),
     (NoiseTV, NoiseTVExperiment, "fused"),
@@ -36,7 +36,16 @@ def test_bits_refuses_what_it_cannot_run():
         engine = "bits"
     with pytest.raises(XenticaException):
         GameOfLifeStatic(StaticBits)
-    class RaggedBits(_Ragged):
+    from xentica_amd import core, seeds
+
+    class NarrowBits(core.Experiment):
+        """Narrower than one 32-cell word."""
+        word = 7
+        size = (20, 40)
         engine = "bits"
+        seed = seeds.patterns.PrimordialSoup(
+            vals={"state": seeds.random.RandInt(0, 1)})
     with pytest.raises(XenticaException):
-        GameOfLife(RaggedBits)
+        GameOfLife(NarrowBits)
+    assert GameOfLife(type("NarrowAuto", (NarrowBits, ),
+                           {"engine": "auto"}))._engine.name == "fused"

@@ -16,6 +16,7 @@ from examples.game_of_life import (
 from examples.workloads import (
     Life512Experiment, Life16kExperiment, LifeLike3D, Life3DExperiment,
     LifeLikeRule, batch_experiments)
+from xentica_amd import core, seeds
 from xentica_amd.runtime import kernels
 from xentica_amd.runtime.bits_engine import BitsEngine
 
@@ -243,3 +244,74 @@ def test_render_3d_projection_vs_plain():
         model.set_viewport((64, 48))
         run(model, 5)
     assert np.array_equal(bits.render(), plain.render())
+
+
+def soup(size, word=5):
+    """A 50 % soup torus of ``size`` (any width)."""
+    return type("Soup%dx%d" % tuple(size), (core.Experiment, ), {
+        "word": word, "size": tuple(size), "zoom": 1, "pos": [0, 0],
+        "seed": seeds.patterns.PrimordialSoup(
+            vals={"state": seeds.random.RandInt(0, 1)})})
+
+
+RAGGED = [(100, 60), (33, 40), (97, 131), (1000, 700)]
+
+
+@pytest.mark.parametrize("size", RAGGED, ids=["%dx%d" % s for s in RAGGED])
+def test_ragged_widths_vs_plain(size):
+    """Widths that are not a multiple of 32 run on padded rows (the batch
+    kernel as a batch of one): cells and colours equal the plain
+    engine's after every group of steps."""
+    plain = GameOfLife(with_engine(soup(size), "plain"))
+    bits = GameOfLife(with_engine(soup(size), "bits"))
+    assert bits._engine.rows is not None
+    for count in (1, 7, 8, 9, 20, 3):
+        run(plain, count)
+        run(bits, count)
+        assert crc(bits) == crc(plain), count
+    assert np.array_equal(bits.gpu.arrays.colors.get(),
+                          plain.gpu.arrays.colors.get())
+
+
+def test_ragged_every_step_vs_oracle():
+    from oracle import oracle as O
+    model = GameOfLife(with_engine(soup((97, 131), 11), "bits"))
+    start = model.gpu.arrays.cells.get()[:model.cells_num]
+    field = O.Field((97, 131), 8, np.uint8, start)
+    want = O.lifelike(field, O.moore_deltas(2), O.moore_rev(2), 8, 12, 60,
+                      colors=False, hashes=True)
+    for step in range(60):
+        model.step()
+        assert crc(model) == want[step], "step %d" % (step + 1)
+
+
+def test_ragged_rule_changes_and_render():
+    exp = [e for e in batch_experiments(count=40, seed=23, side_min=64,
+                                        side_max=300)
+           if e.size[0] % 32][0]
+    plain = LifeLikeRule(with_engine(exp, "plain"))
+    bits = LifeLikeRule(with_engine(exp, "bits"))
+    assert bits._engine.rows is not None
+    rng = np.random.default_rng(9)
+    for _ in range(5):
+        rule = int(rng.integers(0, 1 << 18))
+        plain.rule = rule
+        run(plain, 6)
+        bits.rule = rule
+        run(bits, 6)
+        assert crc(bits) == crc(plain)
+    for model in (plain, bits):
+        model.set_viewport((150, 90))
+    assert np.array_equal(bits.render(), plain.render())
+
+
+def test_ragged_full_size():
+    """A 4010x3000 torus (126 words per row, 22 cells of the last word
+    unused) against the plain engine over 96 steps."""
+    plain = GameOfLife(with_engine(soup((4010, 3000)), "plain"))
+    bits = GameOfLife(with_engine(soup((4010, 3000)), "bits"))
+    assert bits._engine.rows is not None
+    run(plain, 96)
+    run(bits, 96)
+    assert bits._engine.crc32() == kernels.crc32_u8(
+        plain.gpu.arrays.cells.device_ptr(), plain.cells_num)

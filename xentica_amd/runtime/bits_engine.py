@@ -31,11 +31,13 @@ __all__ = ['BitsEngine', 'eligible_size']
 
 
 def eligible_size(size):
-    """Whether the bit-sliced kernels take a field of ``size``."""
+    """Whether the bit-sliced kernels take a field of ``size``: 2-D fields
+    at least 32 cells wide (widths that are not a multiple of 32 run on
+    the batch kernel's padded rows), 3-D ones whose width / 32 divides 64."""
     if len(size) == 2:
         width, height = size
-        return width % 32 == 0 and width > 0 and height > 0 and \
-            width * height // 8 < 1 << 31
+        return width >= 32 and height > 0 and \
+            4 * ((width + 31) // 32) * height < 1 << 31
     if len(size) == 3:
         width, height, depth = size
         return (width % 32 == 0 and width > 0 and 64 % (width // 32) == 0
@@ -54,8 +56,9 @@ class BitsEngine:
         if not eligible_size(model.size):
             raise XenticaException(
                 "bits engine: the field %s does not fit the bit-sliced "
-                "kernels (width a multiple of 32; in 3-D width / 32 "
-                "dividing 64)" % (model.size, ))
+                "kernels (2-D: at least 32 cells wide; 3-D: width a "
+                "multiple of 32 with width / 32 dividing 64)"
+                % (model.size, ))
         self.probe = LifeLikeProbe(model)
         if self.probe.rule(self.probe.values()) is None:
             raise XenticaException(
@@ -65,6 +68,10 @@ class BitsEngine:
         self.depth, self.strips = int(depth), int(strips)
         self.dims = len(model.size)
         self.n = model.cells_num
+        #: 2-D widths that are not a multiple of 32: rows padded to whole
+        #: words, stepped by the batch kernel as a batch of one
+        self.rows = _PaddedRows(model.size, depth) if (
+            self.dims == 2 and model.size[0] % 32) else None
         self._bufs = None
         self._cur = 0
         self._packed = False        # the packed buffer holds the state
@@ -137,7 +144,7 @@ class BitsEngine:
         """
         if self._pending:
             self._launch()
-        if not (self._packed and self._stale):
+        if not (self._packed and self._stale) or self.rows is not None:
             return False
         function = self._render_function()
         if function is None:
@@ -211,14 +218,17 @@ class BitsEngine:
 
     def _buffers(self):
         if self._bufs is None:
-            nbytes = self.n // 8
+            nbytes = self.rows.nbytes if self.rows else self.n // 8
             self._bufs = [hip.Buffer(nbytes), hip.Buffer(nbytes)]
         return self._bufs
 
     def _pack(self):
         bufs = self._buffers()
         plane = self.model.gpu.arrays.cells.plane_ptr(self.n)
-        kernels.bits_pack(plane, bufs[self._cur].ptr, self.n)
+        if self.rows:
+            self.rows.pack(plane, bufs[self._cur].ptr)
+        else:
+            kernels.bits_pack(plane, bufs[self._cur].ptr, self.n)
         self._packed = True
 
     def _launch(self):
@@ -227,7 +237,13 @@ class BitsEngine:
         if not self._packed:
             self._pack()
         bufs = self._buffers()
-        if self.dims == 2:
+        if self.rows:
+            for count in self.rows.launch_plan(steps):
+                self.rows.launch(bufs[self._cur].ptr,
+                                 bufs[1 - self._cur].ptr, count, born,
+                                 sustain)
+                self._cur = 1 - self._cur
+        elif self.dims == 2:
             width, height = self.model.size
             in_b = kernels.life_bits_run(
                 bufs[self._cur].ptr, bufs[1 - self._cur].ptr, width // 32,
@@ -246,9 +262,15 @@ class BitsEngine:
         """The packed state into the main plane and the colours."""
         arrays = self.model.gpu.arrays
         _, _, (dead, alive) = self._pending_rule
-        kernels.bits_expand(self._bufs[self._cur].ptr,
-                            arrays.cells.plane_ptr(self.n),
-                            arrays.colors.device_ptr(), self.n, dead, alive)
+        if self.rows:
+            self.rows.expand(self._bufs[self._cur].ptr,
+                             arrays.cells.plane_ptr(self.n),
+                             arrays.colors.device_ptr(), dead, alive)
+        else:
+            kernels.bits_expand(self._bufs[self._cur].ptr,
+                                arrays.cells.plane_ptr(self.n),
+                                arrays.colors.device_ptr(), self.n, dead,
+                                alive)
         arrays.colors.bounds = tuple(
             (min(a, b), max(a, b)) for a, b in zip(dead, alive))
         self._stale = False
@@ -258,12 +280,97 @@ class BitsEngine:
 
     @property
     def state_ptr(self):
-        """Device address of the packed state (launch queued steps first)."""
+        """Device address of the packed state (launch queued steps first;
+        rows padded to whole words when the width is not a multiple of
+        32)."""
         self.finish()
         if not self._packed:
             self._pack()
         return self._bufs[self._cur].ptr
 
     def crc32(self):
-        """``binascii.crc32`` of the main plane, from the packed state."""
+        """``binascii.crc32`` of the main plane, from the packed state (from
+        the expanded plane for padded rows)."""
+        if self.rows:
+            self.flush()
+            return kernels.crc32_u8(
+                self.model.gpu.arrays.cells.plane_ptr(self.n), self.n)
         return kernels.crc32_bits(self.state_ptr, self.n)
+
+
+class _PaddedRows:
+    """
+    A 2-D torus whose width is not a multiple of 32, packed as rows of
+    ``ceil(width / 32)`` words (the bits past the width zero) and stepped
+    by the batch kernel (``xen_life_batch_launch``) as a batch of one
+    instance: its lanes read 32-cell windows that run over the row's end
+    into the row's start, so the torus wrap is exact for any width.  All
+    calls go to the null stream, in order with the engine's other work.
+    """
+
+    def __init__(self, size, depth):
+        from xentica_amd.runtime import bits_batch
+        self.width, self.height = (int(v) for v in size)
+        self.pitch = bits_batch.pitch_of(self.width)
+        self.nbytes = 4 * self.pitch * self.height
+        self.depth = int(depth)
+        self._lib = None
+        self._rule = None
+
+    def _setup(self):
+        """The lane plan and the instance table on the device (on first
+        use, so that models build without a GPU)."""
+        if self._lib is not None:
+            return
+        from xentica_amd.runtime import bits_batch
+        size = (self.width, self.height)
+        simds = 4 * hip.device().compute_units
+        # the strip height that comes closest to two waves per SIMD (the
+        # batch autotuner's usual pick), else the plan's default
+        rows = bits_batch.balanced_strips([size], simds, per_simd=(2, ))
+        lanes, wave_rows = bits_batch.plan_lanes([size],
+                                                 rows[0] if rows else 48)
+        self.n_waves = len(wave_rows)
+        self._lanes = hip.Buffer(lanes.nbytes)
+        self._lanes.upload(lanes)
+        self._rows = hip.Buffer(wave_rows.nbytes)
+        self._rows.upload(wave_rows)
+        self._table = np.zeros((1, 8), dtype=np.int32)
+        self._table[0, :4] = (0, self.pitch, self.width, self.height)
+        self._table_buf = hip.Buffer(self._table.nbytes)
+        self._lib = bits_batch._lib()
+
+    def launch_plan(self, steps):
+        """Steps per launch of ``steps`` queued steps."""
+        return [self.depth] * (steps // self.depth) + (
+            [steps % self.depth] if steps % self.depth else [])
+
+    def pack(self, plane, bits):
+        """uint8 main plane -> padded packed rows."""
+        self._setup()
+        hip.check(self._lib.xen_bits_pack_rows(
+            plane, bits, self.width, self.height, self.pitch, None),
+            "xen_bits_pack_rows")
+
+    def expand(self, bits, plane, colors, dead, alive):
+        """Padded packed rows -> uint8 main plane and colours by state."""
+        import ctypes
+        self._setup()
+        hip.check(self._lib.xen_bits_expand_rows(
+            bits, plane, colors, self.width, self.height, self.pitch,
+            (ctypes.c_int32 * 3)(*dead), (ctypes.c_int32 * 3)(*alive), None),
+            "xen_bits_expand_rows")
+
+    def launch(self, src, dst, steps, born, sustain):
+        """``steps`` (at most 8) steps from ``src`` into ``dst``."""
+        self._setup()
+        if self._rule != (born, sustain):
+            # the table is read by launches already queued on the same
+            # (null) stream; the copy is ordered after them
+            self._table[0, 4:6] = (born, sustain)
+            self._table_buf.upload(self._table)
+            self._rule = (born, sustain)
+        hip.check(self._lib.xen_life_batch_launch(
+            src, dst, self.nbytes, self._table_buf.ptr, self._lanes.ptr,
+            self._rows.ptr, self.n_waves, steps, None),
+            "xen_life_batch_launch")
