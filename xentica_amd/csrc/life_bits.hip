@@ -234,11 +234,20 @@ int launch_any_k(int k, const uint32_t *src, uint32_t *dst, int W, int H, int ti
     }
 }
 
-// Row strips per launch when the caller passes 0: three waves per SIMD of
-// the device (rounded down, so that no CU receives one workgroup more than
-// the others), and strips at least 4K rows tall.  Three waves per SIMD
-// measured fastest on MI355X at 16384^2: taller strips repeat fewer halo
-// rows, and three waves still hide the load latency.
+// Row strips per launch when the caller passes 0.  A launch is bound by
+// VALU issue, and a strip of S output rows costs (S + 2K) level rows per
+// level, so shorter strips add halo work while more waves per SIMD hide
+// more latency.  For w = 1, 2, 3 waves per SIMD of the device (strip count
+// rounded down, so that no CU receives one workgroup more than the others,
+// and at most one strip per row) the modelled time is
+//   max(1, waves / SIMDs) * (S + 2K) / issue(w),
+// with issue(w) the share of a SIMD's VALU slots that w waves keep busy,
+// measured on MI355X at depth 8: 0.55, 0.85 and 1.  The smallest wins.
+// Measured optima it reproduces: 16384^2 three waves per SIMD (341 strips,
+// 4.6 us per step against 5.6 at 1.5 waves); 4096^2 and 2048^2 one wave
+// per SIMD (1.27 and 0.83 us per step, against 1.89 and 1.78 with strips
+// of at least 4K rows); 512^2 and 640x360 a strip per row (0.74 and 0.76
+// us against 1.78 and 1.80).
 int auto_strips(int tiles_x, int H, int depth) {
     static int simds = 0;
     if (!simds) {
@@ -249,10 +258,23 @@ int auto_strips(int tiles_x, int H, int depth) {
         else
             simds = 1024;
     }
-    long long strips = (3LL * simds) / tiles_x;
-    const long long max_strips = H / (4 * depth) > 0 ? H / (4 * depth) : 1;
-    if (strips > max_strips) strips = max_strips;
-    return strips < 1 ? 1 : static_cast<int>(strips);
+    constexpr double kIssue[4] = {0.0, 0.55, 0.85, 1.0};
+    long long best = 1;
+    double best_time = 0.0;
+    for (int w = 1; w <= 3; ++w) {
+        long long strips = (static_cast<long long>(w) * simds) / tiles_x;
+        if (strips > H) strips = H;
+        if (strips < 1) strips = 1;
+        const double rows = static_cast<double>(H) / static_cast<double>(strips);
+        double per_simd = static_cast<double>(strips * tiles_x) / simds;
+        if (per_simd < 1.0) per_simd = 1.0;
+        const double time = per_simd * (rows + 2.0 * depth) / kIssue[w];
+        if (w == 1 || time < best_time) {
+            best = strips;
+            best_time = time;
+        }
+    }
+    return static_cast<int>(best);
 }
 
 }  // namespace

@@ -104,7 +104,9 @@ class BitsEngine:
         if self._in_fallback:
             self._fallback.flush()
             self._in_fallback = False
-        if self._pending and rule != self._pending_rule:
+        # the probe returns the same rule object for the same values
+        if self._pending and rule is not self._pending_rule and \
+                rule != self._pending_rule:
             self._launch()
         self._pending_rule = rule
         self._pending += 1

@@ -1047,8 +1047,12 @@ class FusedEngine:
 
     def step(self):
         """Queue one step with the current parameter values."""
-        self._setup()
-        values = self._inst.param_values()
+        if self._inst is None:
+            self._setup()
+        # the class-wide values (what _Instances.param_values() gives for a
+        # single model), read without its per-instance lookups
+        params = self.program.params
+        values = [tuple([p.value for p in params])] if params else [()]
         if self._pending and values != self._pending_values:
             self.flush()
         self._pending_values = values

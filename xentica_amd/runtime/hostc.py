@@ -375,7 +375,9 @@ class LifeLikeProbe:
     def values(self):
         """The parameter values a single model's step reads: the class's,
         as the reference's ``step()`` passes them (``base.py:717, 722``)."""
-        return tuple(p.value for p in self.params)
+        if not self.params:
+            return ()
+        return tuple([p.value for p in self.params])
 
     def _stored_colour(self):
         """A stored colour from which every step's colour is its new value."""
@@ -401,7 +403,10 @@ class LifeLikeProbe:
     def rule(self, values):
         """``(born, sustain, colours)`` for these parameter values, or
         ``None`` if the model is not that Life-like rule for them."""
-        values = tuple(values)
+        try:
+            return self._rules[values]
+        except (KeyError, TypeError):  # new values, or not a tuple
+            values = tuple(values)
         if values in self._rules:
             return self._rules[values]
         result = self._probe(values)
