@@ -327,11 +327,8 @@ class _PaddedRows:
         from xentica_amd.runtime import bits_batch
         size = (self.width, self.height)
         simds = 4 * hip.device().compute_units
-        # the strip height that comes closest to two waves per SIMD (the
-        # batch autotuner's usual pick), else the plan's default
-        rows = bits_batch.balanced_strips([size], simds, per_simd=(2, ))
-        lanes, wave_rows = bits_batch.plan_lanes([size],
-                                                 rows[0] if rows else 48)
+        lanes, wave_rows = bits_batch.plan_lanes(
+            [size], self.strip_rows(simds))
         self.n_waves = len(wave_rows)
         self._lanes = hip.Buffer(lanes.nbytes)
         self._lanes.upload(lanes)
@@ -341,6 +338,22 @@ class _PaddedRows:
         self._table[0, :4] = (0, self.pitch, self.width, self.height)
         self._table_buf = hip.Buffer(self._table.nbytes)
         self._lib = bits_batch._lib()
+
+    def strip_rows(self, simds):
+        """Rows per strip by the cost model of the 2-D kernel's default
+        (``auto_strips`` in ``csrc/life_bits.hip``): for one, two and three
+        waves per SIMD, (waves per SIMD) x (strip rows + 2K) over the share
+        of VALU slots that many waves keep busy (0.55, 0.85, 1)."""
+        from xentica_amd.runtime import bits_batch
+        tiles = -(-self.pitch // bits_batch.MAX_TILE_WORDS)
+        best = None
+        for waves, issue in ((1, 0.55), (2, 0.85), (3, 1.0)):
+            strips = max(1, min(self.height, (waves * simds) // tiles))
+            per_simd = max(1.0, strips * tiles / simds)
+            time = per_simd * (self.height / strips + 2 * self.depth) / issue
+            if best is None or time < best[0]:
+                best = (time, strips)
+        return -(-self.height // best[1])
 
     def launch_plan(self, steps):
         """Steps per launch of ``steps`` queued steps."""
