@@ -154,25 +154,31 @@ __device__ __forceinline__ void perceive(const float *__restrict__ x, const Cell
                                          int h, int w, const float *__restrict__ dw,
                                          const float *__restrict__ bdw, float p[4],
                                          float4 nb[kTaps]) {
+    // the nine torus-wrapped neighbour addresses first, then all nine loads,
+    // then the taps in the order dy-major, dx-minor (so that the loads are
+    // in flight together rather than one or two at a time)
+    long long row[3];
+    int col[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int yn = c.y + d - 1;
+        yn = yn < 0 ? yn + h : (yn >= h ? yn - h : yn);
+        row[d] = (c.b * h + yn) * (long long)w;
+        int xn = c.x + d - 1;
+        col[d] = xn < 0 ? xn + w : (xn >= w ? xn - w : xn);
+    }
+#pragma unroll
+    for (int t = 0; t < kTaps; ++t)
+        nb[t] = *reinterpret_cast<const float4 *>(x + (row[t / 3] + col[t % 3]) * kC + 4 * g);
 #pragma unroll
     for (int i = 0; i < 4; ++i) p[i] = bdw[4 * g + i];
 #pragma unroll
-    for (int dy = -1; dy <= 1; ++dy) {
-        int yn = c.y + dy;
-        yn = yn < 0 ? yn + h : (yn >= h ? yn - h : yn);
-#pragma unroll
-        for (int dx = -1; dx <= 1; ++dx) {
-            int xn = c.x + dx;
-            xn = xn < 0 ? xn + w : (xn >= w ? xn - w : xn);
-            const int t = (dy + 1) * 3 + (dx + 1);
-            const float4 v = *reinterpret_cast<const float4 *>(
-                x + ((c.b * h + yn) * (long long)w + xn) * kC + 4 * g);
-            nb[t] = v;
-            p[0] = fmaf(dw[(4 * g + 0) * kTaps + t], v.x, p[0]);
-            p[1] = fmaf(dw[(4 * g + 1) * kTaps + t], v.y, p[1]);
-            p[2] = fmaf(dw[(4 * g + 2) * kTaps + t], v.z, p[2]);
-            p[3] = fmaf(dw[(4 * g + 3) * kTaps + t], v.w, p[3]);
-        }
+    for (int t = 0; t < kTaps; ++t) {
+        const float4 v = nb[t];
+        p[0] = fmaf(dw[(4 * g + 0) * kTaps + t], v.x, p[0]);
+        p[1] = fmaf(dw[(4 * g + 1) * kTaps + t], v.y, p[1]);
+        p[2] = fmaf(dw[(4 * g + 2) * kTaps + t], v.z, p[2]);
+        p[3] = fmaf(dw[(4 * g + 3) * kTaps + t], v.w, p[3]);
     }
 }
 
