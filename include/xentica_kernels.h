@@ -31,9 +31,11 @@ extern "C" {
  * State: 1 bit per cell, 32 cells per uint32 word, x0 fastest;
  * width_words = width / 32.  One launch advances `depth` (1..8) steps;
  * the rows are split into `strips` balanced strips, one wavefront per
- * (62-word column tile, strip); strips = 0 picks one, two or three waves
- * per SIMD (at most one strip per row) by the library's cost model.
- * The field must be smaller than 2 GiB. */
+ * (62-word column tile, strip); the words left over after the whole
+ * tiles run in row segments of ceil(height / strips) rows, up to four
+ * interior segments sharing a wavefront.  strips = 0 picks one, two or
+ * three waves per SIMD (at most one strip per row) by the library's cost
+ * model.  The field must be smaller than 2 GiB. */
 int xen_life_bits_launch(const void *src, void *dst, int width_words, int height,
                          uint32_t born, uint32_t sustain, int depth, int strips,
                          xen_stream stream);

@@ -2,10 +2,12 @@
 CPU emulation of ``life_bits_kernel`` (``xentica_amd/csrc/life_bits.hip``).
 
 Test infrastructure: it mirrors the kernel's control flow line by line
-(balanced column/row tiles, the K-level row pipeline with its prologue,
-steady state unrolled by three and remainder, the wave-uniform row
-wrapping, DPP wave shifts with zero fill at the wave edges, and stores
-dropped by the buffer range check) with the 64 lanes of a wavefront as
+(the wave plan: whole 62-column tiles over balanced row strips and the
+remainder column's segments, interior ones sharing a wave; the K-level
+row pipeline with its prologue, steady state of nine rows requested a
+whole iteration ahead and remainder, the row wrapping, DPP wave shifts
+with zero fill at the wave edges, and stores dropped by the buffer range
+check) with the 64 lanes of a wavefront as
 NumPy vectors, so the kernel's termination, bounds and results can be
 checked on a machine without a GPU.
 """
@@ -57,71 +59,147 @@ def life_word(a, b, c, born, sustain, fast):
         res |= eq & ((alive & live_next) | ((~alive & M32) & dead_next))
     return res
 
-def kernel(src, W, H, tiles_x, tiles_y, K, born, sustain):
+OUT_LANES = 62
+AHEAD = 9
+MAX_GROUP = 4
+
+
+def make_plan(W, H, K, strips):
+    """``make_plan``: whole 62-column tiles over balanced row strips, and the
+    remainder column's segments, interior ones packed ``group`` to a wave."""
+    p = dict(full=W // OUT_LANES, rem=W % OUT_LANES, strips=strips, group=0,
+             seg_rows=0, segs=0, e_lo=0, e_hi=0)
+    p["waves"] = p["full"] * strips
+    if p["rem"] == 0:
+        return p
+    p["group"] = min(64 // (p["rem"] + 2), MAX_GROUP)
+    s = p["seg_rows"] = -(-H // strips)
+    p["segs"] = -(-H // s)
+    if p["group"] >= 2:
+        e_lo = min(-(-K // s), p["segs"])
+        hi = (H - K - AHEAD) // s if H - K - AHEAD >= 0 else 0
+        p["e_lo"], p["e_hi"] = e_lo, min(max(hi, e_lo), p["segs"])
+    shared = p["e_hi"] - p["e_lo"]
+    p["waves"] += (p["segs"] - shared) + -(-shared // max(p["group"], 1))
+    return p
+
+
+def wave_tile(p, wave, H):
+    """(col_lo, ncols, row_lo, out, seg_lanes, nsegs) of a wave."""
+    whole = p["full"] * p["strips"]
+    if wave < whole:
+        tx, ty = wave % p["full"], wave // p["full"]
+        row_lo = ty * H // p["strips"]
+        return (tx * OUT_LANES, OUT_LANES, row_lo,
+                (ty + 1) * H // p["strips"] - row_lo, 64, 1)
+    r = wave - whole
+    shared = p["e_hi"] - p["e_lo"]
+    solo = p["segs"] - shared
+    seg_lanes, nsegs = 64, 1
+    if r < solo:
+        seg = r if r < p["e_lo"] else p["e_hi"] + (r - p["e_lo"])
+    else:
+        seg = p["e_lo"] + (r - solo) * p["group"]
+        seg_lanes = p["rem"] + 2
+        nsegs = min(p["e_hi"] - seg, p["group"])
+    row_lo = seg * p["seg_rows"]
+    return (p["full"] * OUT_LANES, p["rem"], row_lo,
+            min(H - row_lo, p["seg_rows"]), seg_lanes, nsegs)
+
+
+def kernel(src, W, H, plan, K, born, sustain, writes):
     fast = (born & 0x1FF) == 8 and (sustain & 0x1FF) == 12
     dst = np.full(W * H, 0xDEADBEEF, dtype=np.uint64)
     nbytes = W * H * 4
+    row_bytes = W * 4
     lane = np.arange(64)
-    for wave in range(tiles_x * tiles_y):
-        tx, ty = wave % tiles_x, wave // tiles_x
-        col_lo = tx * W // tiles_x; col_hi = (tx + 1) * W // tiles_x
-        assert col_hi - col_lo <= 62
-        col = wrap(col_lo - 1 + lane, W)
-        row_lo = ty * H // tiles_y; row_hi = (ty + 1) * H // tiles_y
-        out = row_hi - row_lo; assert out >= 1
+    for wave in range(plan["waves"]):
+        col_lo, ncols, row_lo, out, seg_lanes, nsegs = wave_tile(plan, wave, H)
+        assert 1 <= ncols <= OUT_LANES and out >= 1
+        seg = lane // seg_lanes
+        sl = lane - seg * seg_lanes
+        active = seg < nsegs
+        seg_off = seg * plan["seg_rows"] * row_bytes
+        col = wrap(col_lo - 1 + sl, W)
+        lane_off = np.where(active, col * 4 + seg_off, nbytes)
+        out_col = col_lo - 1 + sl
+        lane_out = active & (sl >= 1) & (sl <= ncols)
+        out_voff = np.where(lane_out, out_col * 4 + seg_off, nbytes)
         L = out + 2 * K; row0 = row_lo - K
-        out_col = col_lo - 1 + lane
-        lane_out = (lane >= 1) & (out_col < col_hi)
-        out_voff = np.where(lane_out, out_col * 4, nbytes)
-        def load_row(i):
-            soff = wrap(row0 + i, H) * W * 4
-            off = col * 4 + soff
-            assert np.all(off < nbytes)
-            return src[off // 4].copy()
+        def load_row(i, fed=True):
+            soff = wrap(row0 + i, H) * row_bytes
+            off = lane_off + soff
+            # every load of an active lane is in the field (no reliance on
+            # the range check's treatment of the scalar offset); idle lanes
+            # carry a vector offset past the end and read zero
+            assert np.all(off[active] < nbytes), (wave, i)
+            val = np.zeros(64, dtype=np.uint64)
+            val[active] = src[off[active] // 4]
+            if fed and nsegs > 1:
+                # a shared wave's segments never wrap their rows
+                assert 0 <= row0 + i < H
+            return val
         def store_row(val, r):
             keep = r < out
             voff = out_voff if keep else np.full(64, nbytes)
-            soff = (row_lo + r) * W * 4 if keep else 0
+            soff = (row_lo + r) * row_bytes if keep else 0
             for ln in range(64):
                 if voff[ln] < nbytes:   # range check on the vector offset
                     off = voff[ln] + soff
                     assert off < nbytes, (wave, r, ln)
                     dst[off // 4] = val[ln]
+                    writes[off // 4] += 1
         lev = [[None] * 3 for _ in range(K + 1)]
         def feed(i, inp, all_levels):
+            assert i < L
             lev[0][i % 3] = hrow(inp)
             for l in range(1, K + 1):
                 if all_levels or i >= 2 * l:
                     j3 = (i - l) % 3
                     v = life_word(lev[l-1][(j3 + 2) % 3], lev[l-1][j3], lev[l-1][(j3 + 1) % 3], born, sustain, fast)
                     lev[l][j3] = hrow(v) if l < K else (v, None, None)
-        for i in range(2 * K):
-            feed(i, load_row(i), False)
+        # prologue: 2K rows, then the first AHEAD rows requested
+        pro = [load_row(r) for r in range(2 * K)]
         i = 2 * K
-        a, b, c = load_row(i), load_row(i + 1), load_row(i + 2)
-        na, nb, nc = load_row(i + 3), load_row(i + 4), load_row(i + 5)
+        cur = [load_row(i + r, fed=i + r < L) for r in range(AHEAD)]
+        for r in range(2 * K):
+            feed(r, pro[r], False)
         iters = 0
-        while i + 2 < L:
-            fa, fb, fc = load_row(i + 6), load_row(i + 7), load_row(i + 8)
-            for q, v in enumerate((a, b, c)):
-                feed(i + q, v, True)
+        while i + AHEAD - 1 < L:
+            nxt = [load_row(i + AHEAD + r, fed=i + AHEAD + r < L)
+                   for r in range(AHEAD)]
+            for q in range(AHEAD):
+                feed(i + q, cur[q], True)
                 store_row(lev[K][(i + q - K) % 3][0], i + q - 2 * K)
-            a, b, c = na, nb, nc
-            na, nb, nc = fa, fb, fc
-            i += 3; iters += 1
+            cur = nxt
+            i += AHEAD; iters += 1
             assert iters < 100000
-        if i < L:
-            feed(i, a, True); store_row(lev[K][(i - K) % 3][0], i - 2 * K)
-            if i + 1 < L:
-                feed(i + 1, b, True); store_row(lev[K][(i + 1 - K) % 3][0], i + 1 - 2 * K)
+        for r in range(AHEAD):
+            if i + r >= L:
+                break
+            feed(i + r, cur[r], True)
+            store_row(lev[K][(i + r - K) % 3][0], i + r - 2 * K)
     assert not np.any(dst == 0xDEADBEEF), "unwritten words"
     return dst
 
-def auto_strips(tiles_x, H, depth, simds=1024):
-    strips = (4 * simds) // tiles_x
-    max_strips = H // (4 * depth) if H // (4 * depth) > 0 else 1
-    strips = min(strips, max_strips)
-    return max(1, strips)
+
+def auto_strips(W, H, depth, simds=1024):
+    """``auto_strips``: the strip count of the smallest modelled time."""
+    issue = (0.0, 0.55, 0.85, 1.0)
+    one = make_plan(W, H, depth, 1)
+    per_strip = one["full"] + (1.0 / max(one["group"], 1) if one["rem"] else 0)
+    best, best_time = 1, None
+    for w in (1, 2, 3):
+        target = w * simds
+        strips = min(int(target / per_strip) + 1, H)
+        while strips > 1 and make_plan(W, H, depth, strips)["waves"] > target:
+            strips -= 1
+        waves = make_plan(W, H, depth, strips)["waves"]
+        t = max(1.0, waves / simds) * (H / strips + 2 * depth) / issue[w]
+        if best_time is None or t < best_time:
+            best, best_time = strips, t
+    return best
+
 
 def pack(cells, W, H):
     bits = np.zeros(W * H, dtype=np.uint64)
@@ -141,13 +219,14 @@ def unpack(bits, W, H):
 def run(cells, width, height, depth, born, sustain, steps, strips=0):
     """Run the emulated kernel launch sequence; returns the uint8 plane."""
     W, H = width // 32, height
-    tiles_x = (W + 61) // 62
-    tiles_y = strips if strips > 0 else auto_strips(tiles_x, H, depth)
-    tiles_y = min(tiles_y, H)
     bits = pack(cells, W, H)
     left = steps
     while left > 0:
         k = min(depth, left)
-        bits = kernel(bits, W, H, tiles_x, tiles_y, k, born, sustain)
+        rows = strips if strips > 0 else auto_strips(W, H, k)
+        plan = make_plan(W, H, k, min(rows, H))
+        writes = np.zeros(W * H, dtype=np.int64)
+        bits = kernel(bits, W, H, plan, k, born, sustain, writes)
+        assert np.all(writes == 1), "a word stored twice or not at all"
         left -= k
     return unpack(bits, W, H)

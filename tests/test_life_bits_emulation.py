@@ -48,3 +48,74 @@ def test_strip_heights_around_depth(height, strips):
     of every length modulo four."""
     _check(64, height, 8, 8, 12, 8, strips=strips)
     _check(64, height, 8, 6, 20, 8, strips=strips, seed=1)
+
+
+@pytest.mark.parametrize("words,height,depth,strips", [
+    (67, 120, 4, 12),    # one whole tile + 5 columns: 4 segments per wave
+    (16, 96, 3, 32),     # no whole tile: 16 columns, 3 segments per wave
+    (63, 150, 8, 10),    # 1 column left: 4 segments per wave
+    (140, 90, 2, 9),     # two whole tiles + 16 columns
+    (80, 77, 5, 7),      # 18 columns, segments of 11 rows (the last 0)
+    (84, 60, 2, 20)])    # 22 columns: 2 segments per wave
+def test_shared_remainder_waves(words, height, depth, strips):
+    """Interior segments of the remainder column share waves; every word is
+    stored once, every load of an active lane is in the field, and the
+    result is the oracle's."""
+    plan = E.make_plan(words, height, depth, strips)
+    assert plan["e_hi"] - plan["e_lo"] >= 2 and plan["group"] >= 2
+    _check(32 * words, height, depth, 8, 12, depth + 1, strips=strips)
+    _check(32 * words, height, depth, 6, 20, depth, strips=strips, seed=2)
+
+
+def test_wide_remainder_runs_one_segment_per_wave():
+    """31 or more columns left over: no sharing (one segment per wave)."""
+    plan = E.make_plan(93, 60, 5, 6)
+    assert plan["group"] == 1 and plan["e_hi"] == plan["e_lo"]
+    _check(32 * 93, 60, 5, 8, 12, 7, strips=6)
+
+
+def test_plan_of_the_16384_field():
+    """16384^2 at depth 8: 8 whole tiles and 16 columns per strip; the
+    default plan fills at most three waves per SIMD, and 95 % or more of
+    its lanes compute output words (89 % with nine 57-column tiles)."""
+    strips = E.auto_strips(512, 16384, 8)
+    plan = E.make_plan(512, 16384, 8, strips)
+    assert plan["full"] == 8 and plan["rem"] == 16 and plan["group"] == 3
+    assert 3000 <= plan["waves"] <= 3 * 1024
+    out_lanes = 512 * 16384 / (16384 / strips)   # output word-columns x strips
+    assert out_lanes / (64 * plan["waves"]) >= 0.95
+
+
+def _plan_coverage(words, height, depth, strips):
+    """Store count of every word and the loaded row range of every wave's
+    active segments under the plan (the kernel's offsets, without the
+    row pipeline)."""
+    plan = E.make_plan(words, height, depth, strips)
+    count = np.zeros((height, words), dtype=np.int32)
+    for wave in range(plan["waves"]):
+        col_lo, ncols, row_lo, out, seg_lanes, nsegs = E.wave_tile(
+            plan, wave, height)
+        assert 1 <= ncols <= E.OUT_LANES and 1 <= out
+        assert nsegs * seg_lanes <= 64
+        last = out + 2 * depth + E.AHEAD - 1     # rows requested: 0 .. last
+        for g in range(nsegs):
+            lo = row_lo + g * plan["seg_rows"]
+            if nsegs > 1:
+                # shared segments: the whole window, rows requested ahead
+                # included, lies in the field without wrapping
+                assert out == plan["seg_rows"]
+                assert lo - depth >= 0 and lo - depth + last <= height - 1
+            count[lo:lo + out, col_lo:col_lo + ncols] += 1
+    return count
+
+
+@pytest.mark.parametrize("words", [1, 2, 15, 16, 31, 33, 61, 62, 63, 67, 93,
+                                   124, 140, 512])
+@pytest.mark.parametrize("height", [1, 7, 18, 37, 90, 361, 4096])
+def test_plan_stores_every_word_once(words, height):
+    for depth in (1, 4, 8):
+        strips_list = {1, 3, min(height, 64), height,
+                       E.auto_strips(words, height, depth)}
+        for strips in sorted(s for s in strips_list if s <= height):
+            count = _plan_coverage(words, height, depth, strips)
+            assert np.all(count == 1), (words, height, depth, strips)

@@ -9,10 +9,15 @@
 // Moore neighbours on a torus (xentica/core/topology/border.py:104-117).
 //
 // Kernel structure (one wavefront per tile, no LDS, no barriers):
-//   * lane l owns one word column; the wave spans 64 adjacent columns of
+//   * lane l owns one word column; a wave spans 64 adjacent columns of
 //     which the 62 inner ones are output (lanes 0 and 63 are halo: their
 //     results go stale one bit per step from the outer edge, so up to 31
 //     steps never reach an inner lane);
+//   * the columns left over after the whole 62-column tiles (fewer than 62)
+//     form a remainder column whose segments are narrower than a wave:
+//     interior row segments of it share a wave, one group of rem + 2 lanes
+//     each, so a 16384-cell row (512 words = 8 x 62 + 16) runs on 8 1/3
+//     waves per row strip instead of 9;
 //   * each lane keeps R = OUT + 2K rows of its column in VGPRs and applies
 //     K steps in registers (temporal blocking: K steps per HBM round trip);
 //   * vertical neighbour sums come from registers, horizontal ones from the
@@ -32,6 +37,15 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kOutLanes = kWave - 2;
 constexpr int kWavesPerGroup = 4;
+// At most this many remainder segments share a wave: a wave's loads and
+// stores then touch that many rows each, and on MI355X groups of 8 and 16
+// made those waves the launch's slowest (2048^2: 0.90 and 1.07 us per step
+// against 0.83 at 4 and 0.79 with none shared; 16384^2 gains 6 % at 3)
+constexpr int kMaxGroup = 4;
+// rows requested ahead of the row being fed: the steady state loads the
+// nine rows of the next iteration, so a wave reads up to row L + 8 of its
+// window of L rows
+constexpr int kAhead = 9;
 // dword 3 of a raw buffer resource on gfx9-family (CDNA) parts
 constexpr int kRsrcFlags = 0x00020000;
 // keep several waves per SIMD resident (caps VGPRs at 512 / this)
@@ -75,49 +89,111 @@ __device__ __forceinline__ uint32_t level_k_row(const Window<K> &w) {
     return w.lev[K][((I3 - K) % 3 + 3) % 3].raw;
 }
 
-// One launch = K steps over the whole field.  Wave w owns tile (tx, ty):
-// output word columns [col_lo, col_hi) (at most 62) and output rows
-// [row_lo, row_hi); both splits are balanced (sizes differ by at most
-// one).  It streams the L = out + 2K rows around them
-// through a K-level row pipeline: when input row i arrives, level l
-// computes row i - l from rows i-l-1 .. i-l+1 of level l-1, so level l
-// holds valid rows [l, L-1-l] and only three rows per level are live.
+// Wave plan of one launch.  Waves [0, full * strips) are the whole
+// 62-column tiles over `strips` balanced row strips.  The rem < 62 columns
+// left of the row form the remainder column, cut into `segs` row segments
+// of seg_rows rows (the last one shorter).  Segments [e_lo, e_hi) are
+// interior: their window of rows, including the rows requested ahead,
+// lies inside the field, so they need no row wrap, and they are packed
+// `group` (at most kMaxGroup) to a wave, each in rem + 2 lanes at its own
+// row offset.  The
+// other segments run one to a wave as before.
+struct Plan {
+    int full, strips, rem, group, seg_rows, segs, e_lo, e_hi;
+    long long waves;
+};
+
+// The plan for `strips` row strips of the whole tiles (1 <= strips <= H).
+Plan make_plan(int W, int H, int K, int strips) {
+    Plan p{};
+    p.full = W / kOutLanes;
+    p.rem = W - p.full * kOutLanes;
+    p.strips = strips;
+    p.waves = static_cast<long long>(p.full) * strips;
+    if (p.rem == 0) return p;
+    p.group = kWave / (p.rem + 2);
+    if (p.group > kMaxGroup) p.group = kMaxGroup;
+    p.seg_rows = (H + strips - 1) / strips;
+    p.segs = (H + p.seg_rows - 1) / p.seg_rows;
+    if (p.group >= 2) {
+        // interior: j * seg_rows >= K and (j + 1) * seg_rows + K + kAhead <= H
+        p.e_lo = (K + p.seg_rows - 1) / p.seg_rows;
+        if (p.e_lo > p.segs) p.e_lo = p.segs;
+        const int hi = H - K - kAhead >= 0 ? (H - K - kAhead) / p.seg_rows : 0;
+        p.e_hi = hi < p.e_lo ? p.e_lo : (hi > p.segs ? p.segs : hi);
+    }
+    const int shared = p.e_hi - p.e_lo;
+    p.waves += (p.segs - shared) + (shared + p.group - 1) / (p.group > 0 ? p.group : 1);
+    return p;
+}
+
+// One launch = K steps over the whole field.  Wave w owns a tile of
+// output word columns [col_lo, col_lo + ncols) and output rows
+// [row_lo, row_lo + out) (see Plan; a shared remainder wave owns nsegs
+// such tiles, segment g at rows row_lo + g * seg_rows).  It streams the
+// L = out + 2K rows around them through a K-level row pipeline: when input
+// row i arrives, level l computes row i - l from rows i-l-1 .. i-l+1 of
+// level l-1, so level l holds valid rows [l, L-1-l] and only three rows per
+// level are live.
 template <int K, bool kFast>
 __global__ __launch_bounds__(kWave * kWavesPerGroup)
 __attribute__((amdgpu_waves_per_eu(kMinWavesPerSimd)))
 void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
-                      int W, int H, int tiles_x, int tiles_y,
-                      uint32_t born, uint32_t sustain) {
+                      int W, int H, Plan plan, uint32_t born, uint32_t sustain) {
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform tile coordinates live in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(
         static_cast<int>(blockIdx.x) * kWavesPerGroup + (static_cast<int>(threadIdx.x) >> 6));
-    if (wave >= tiles_x * tiles_y) return;
-    const int tx = wave % tiles_x;
-    const int ty = wave / tiles_x;
-    // balanced split of the W output columns over tiles_x tiles
-    const int col_lo = static_cast<int>((static_cast<long long>(tx) * W) / tiles_x);
-    const int col_hi = static_cast<int>((static_cast<long long>(tx + 1) * W) / tiles_x);
-    const int col = wrap(col_lo - 1 + lane, W);
-    const int row_lo = static_cast<int>((static_cast<long long>(ty) * H) / tiles_y);
-    const int row_hi = static_cast<int>((static_cast<long long>(ty + 1) * H) / tiles_y);
-    const int out = row_hi - row_lo;
+    if (wave >= plan.waves) return;
+    const int whole = plan.full * plan.strips;
+    int col_lo, ncols, row_lo, out, seg_lanes = kWave, nsegs = 1;
+    if (wave < whole) {
+        const int tx = wave % plan.full;
+        const int ty = wave / plan.full;
+        col_lo = tx * kOutLanes;
+        ncols = kOutLanes;
+        // balanced split of the H rows over the strips
+        row_lo = static_cast<int>((static_cast<long long>(ty) * H) / plan.strips);
+        out = static_cast<int>((static_cast<long long>(ty + 1) * H) / plan.strips) - row_lo;
+    } else {
+        const int r = wave - whole;
+        const int shared = plan.e_hi - plan.e_lo;
+        const int solo = plan.segs - shared;
+        col_lo = plan.full * kOutLanes;
+        ncols = plan.rem;
+        int seg;
+        if (r < solo) {
+            seg = r < plan.e_lo ? r : plan.e_hi + (r - plan.e_lo);
+        } else {
+            seg = plan.e_lo + (r - solo) * plan.group;
+            seg_lanes = plan.rem + 2;
+            nsegs = plan.e_hi - seg < plan.group ? plan.e_hi - seg : plan.group;
+        }
+        row_lo = seg * plan.seg_rows;
+        out = (H - row_lo < plan.seg_rows ? H - row_lo : plan.seg_rows);
+    }
     const int L = out + 2 * K;
     const int row0 = row_lo - K;
 
     // Buffer resources: scalar base and row offsets, one per-lane column
-    // offset.  Out-of-range offsets are dropped by the hardware range
-    // check, which is how masked-off stores are expressed.
+    // offset (plus the segment's row offset in a shared wave).
+    // Out-of-range offsets are dropped by the hardware range check, which
+    // is how masked-off stores and idle lanes are expressed.
     const unsigned bytes = static_cast<unsigned>(static_cast<size_t>(W) * H * 4u);
     const __amdgpu_buffer_rsrc_t in_rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(src), 0, bytes, kRsrcFlags);
     const __amdgpu_buffer_rsrc_t out_rsrc =
         __builtin_amdgcn_make_buffer_rsrc(dst, 0, bytes, kRsrcFlags);
     const int row_bytes = W * 4;
-    const int lane_off = col * 4;
-    const int out_col = col_lo - 1 + lane;
-    const bool lane_out = lane >= 1 && out_col < col_hi;
-    const int out_voff = lane_out ? out_col * 4 : static_cast<int>(bytes);
+    const int seg = lane / seg_lanes;
+    const int sl = lane - seg * seg_lanes;  // lane within its segment
+    const bool active = seg < nsegs;
+    const int seg_off = seg * plan.seg_rows * row_bytes;
+    const int col = wrap(col_lo - 1 + sl, W);
+    const int lane_off = active ? col * 4 + seg_off : static_cast<int>(bytes);
+    const int out_col = col_lo - 1 + sl;
+    const bool lane_out = active && sl >= 1 && sl <= ncols;
+    const int out_voff = lane_out ? out_col * 4 + seg_off : static_cast<int>(bytes);
     const int out_row0 = row_lo;
     const int out_rows = out;
 
@@ -209,27 +285,27 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
 }
 
 template <int K, bool kFast>
-int launch_k(const uint32_t *src, uint32_t *dst, int W, int H, int tiles_x, int tiles_y,
+int launch_k(const uint32_t *src, uint32_t *dst, int W, int H, const Plan &plan,
              uint32_t born, uint32_t sustain, hipStream_t stream) {
-    const long long waves = static_cast<long long>(tiles_x) * tiles_y;
-    const unsigned groups = static_cast<unsigned>((waves + kWavesPerGroup - 1) / kWavesPerGroup);
+    const unsigned groups =
+        static_cast<unsigned>((plan.waves + kWavesPerGroup - 1) / kWavesPerGroup);
     hipLaunchKernelGGL((life_bits_kernel<K, kFast>), dim3(groups), dim3(kWave * kWavesPerGroup),
-                       0, stream, src, dst, W, H, tiles_x, tiles_y, born, sustain);
+                       0, stream, src, dst, W, H, plan, born, sustain);
     return xen::check(hipGetLastError(), "life_bits_kernel launch");
 }
 
 template <bool kFast>
-int launch_any_k(int k, const uint32_t *src, uint32_t *dst, int W, int H, int tiles_x,
-                 int tiles_y, uint32_t born, uint32_t sustain, hipStream_t stream) {
+int launch_any_k(int k, const uint32_t *src, uint32_t *dst, int W, int H, const Plan &plan,
+                 uint32_t born, uint32_t sustain, hipStream_t stream) {
     switch (k) {
-        case 1: return launch_k<1, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
-        case 2: return launch_k<2, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
-        case 3: return launch_k<3, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
-        case 4: return launch_k<4, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
-        case 5: return launch_k<5, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
-        case 6: return launch_k<6, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
-        case 7: return launch_k<7, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
-        case 8: return launch_k<8, kFast>(src, dst, W, H, tiles_x, tiles_y, born, sustain, stream);
+        case 1: return launch_k<1, kFast>(src, dst, W, H, plan, born, sustain, stream);
+        case 2: return launch_k<2, kFast>(src, dst, W, H, plan, born, sustain, stream);
+        case 3: return launch_k<3, kFast>(src, dst, W, H, plan, born, sustain, stream);
+        case 4: return launch_k<4, kFast>(src, dst, W, H, plan, born, sustain, stream);
+        case 5: return launch_k<5, kFast>(src, dst, W, H, plan, born, sustain, stream);
+        case 6: return launch_k<6, kFast>(src, dst, W, H, plan, born, sustain, stream);
+        case 7: return launch_k<7, kFast>(src, dst, W, H, plan, born, sustain, stream);
+        case 8: return launch_k<8, kFast>(src, dst, W, H, plan, born, sustain, stream);
         default: return xen::fail("life_bits: temporal depth must be 1..8", 22);
     }
 }
@@ -237,18 +313,19 @@ int launch_any_k(int k, const uint32_t *src, uint32_t *dst, int W, int H, int ti
 // Row strips per launch when the caller passes 0.  A launch is bound by
 // VALU issue, and a strip of S output rows costs (S + 2K) level rows per
 // level, so shorter strips add halo work while more waves per SIMD hide
-// more latency.  For w = 1, 2, 3 waves per SIMD of the device (strip count
-// rounded down, so that no CU receives one workgroup more than the others,
-// and at most one strip per row) the modelled time is
+// more latency.  For w = 1, 2, 3 waves per SIMD of the device (the most
+// strips whose plan has at most w waves per SIMD, so that no CU receives
+// one workgroup more than the others, and at most one strip per row) the
+// modelled time is
 //   max(1, waves / SIMDs) * (S + 2K) / issue(w),
 // with issue(w) the share of a SIMD's VALU slots that w waves keep busy,
 // measured on MI355X at depth 8: 0.55, 0.85 and 1.  The smallest wins.
-// Measured optima it reproduces: 16384^2 three waves per SIMD (341 strips,
-// 4.6 us per step against 5.6 at 1.5 waves); 4096^2 and 2048^2 one wave
-// per SIMD (1.27 and 0.83 us per step, against 1.89 and 1.78 with strips
-// of at least 4K rows); 512^2 and 640x360 a strip per row (0.74 and 0.76
-// us against 1.78 and 1.80).
-int auto_strips(int tiles_x, int H, int depth) {
+// Measured optima of the model with whole-row tiles (no shared remainder
+// waves): 16384^2 three waves per SIMD (4.6 us per step against 5.6 at 1.5
+// waves); 4096^2 and 2048^2 one wave per SIMD (1.27 and 0.83 us per step,
+// against 1.89 and 1.78 with strips of at least 4K rows); 512^2 and
+// 640x360 a strip per row (0.74 and 0.76 us against 1.78 and 1.80).
+int auto_strips(int W, int H, int depth) {
     static int simds = 0;
     if (!simds) {
         int dev = 0, cus = 256;
@@ -259,14 +336,20 @@ int auto_strips(int tiles_x, int H, int depth) {
             simds = 1024;
     }
     constexpr double kIssue[4] = {0.0, 0.55, 0.85, 1.0};
+    const Plan one = make_plan(W, H, depth, 1);
+    // waves per strip, for the first guess
+    const double per_strip = one.full + (one.rem ? 1.0 / (one.group > 1 ? one.group : 1) : 0.0);
     long long best = 1;
     double best_time = 0.0;
     for (int w = 1; w <= 3; ++w) {
-        long long strips = (static_cast<long long>(w) * simds) / tiles_x;
+        const long long target = static_cast<long long>(w) * simds;
+        long long strips = static_cast<long long>(target / per_strip) + 1;
         if (strips > H) strips = H;
-        if (strips < 1) strips = 1;
+        while (strips > 1 && make_plan(W, H, depth, static_cast<int>(strips)).waves > target)
+            --strips;
+        const long long waves = make_plan(W, H, depth, static_cast<int>(strips)).waves;
         const double rows = static_cast<double>(H) / static_cast<double>(strips);
-        double per_simd = static_cast<double>(strips * tiles_x) / simds;
+        double per_simd = static_cast<double>(waves) / simds;
         if (per_simd < 1.0) per_simd = 1.0;
         const double time = per_simd * (rows + 2.0 * depth) / kIssue[w];
         if (w == 1 || time < best_time) {
@@ -290,18 +373,16 @@ int xen_life_bits_launch(const void *src, void *dst, int width_words, int height
     if (static_cast<long long>(width_words) * height * 4 >= (1LL << 31))
         return xen::fail("xen_life_bits_launch: field must be smaller than 2 GiB", 22);
     if (depth < 1 || depth > 8) return xen::fail("xen_life_bits_launch: depth must be 1..8", 22);
-    const int tiles_x = (width_words + kOutLanes - 1) / kOutLanes;
-    int tiles_y = strips > 0 ? strips : auto_strips(tiles_x, height, depth);
-    if (tiles_y > height) tiles_y = height;
+    int rows = strips > 0 ? strips : auto_strips(width_words, height, depth);
+    if (rows > height) rows = height;
+    const Plan plan = make_plan(width_words, height, depth, rows);
     auto s = reinterpret_cast<hipStream_t>(stream);
     auto in = static_cast<const uint32_t *>(src);
     auto out = static_cast<uint32_t *>(dst);
     const bool fast = (born & 0x1FFu) == 0x8u && (sustain & 0x1FFu) == 0xCu;
     if (fast)
-        return launch_any_k<true>(depth, in, out, width_words, height, tiles_x, tiles_y, born,
-                                  sustain, s);
-    return launch_any_k<false>(depth, in, out, width_words, height, tiles_x, tiles_y, born,
-                               sustain, s);
+        return launch_any_k<true>(depth, in, out, width_words, height, plan, born, sustain, s);
+    return launch_any_k<false>(depth, in, out, width_words, height, plan, born, sustain, s);
 }
 
 int xen_life_bits_run(void *buf_a, void *buf_b, int width_words, int height,
