@@ -62,18 +62,39 @@ def life_word(a, b, c, born, sustain, fast):
 OUT_LANES = 62
 AHEAD = 9
 MAX_GROUP = 4
+G = 4           # waves per workgroup
+
+
+def exchange_fits(H, K, groups):
+    """``exchange_fits``: every strip of a group but the last has at least
+    2K rows, the last at least one."""
+    if groups < 1 or groups > H:
+        return False
+    rmin = H // groups
+    rmax = rmin + 1 if H % groups else rmin
+    for R in range(rmin, rmax + 1):
+        hx = (R + K - 1) // G
+        if hx < 2 * K or R - (G - 1) * hx < 1:
+            return False
+    return True
 
 
 def make_plan(W, H, K, strips):
-    """``make_plan``: whole 62-column tiles over balanced row strips, and the
-    remainder column's segments, interior ones packed ``group`` to a wave."""
+    """``make_plan``: whole 62-column tiles over row strips (exchanging
+    groups of G strips where they fit), and the remainder column's
+    segments, interior ones packed ``group`` to a wave."""
     p = dict(full=W // OUT_LANES, rem=W % OUT_LANES, strips=strips, group=0,
-             seg_rows=0, segs=0, e_lo=0, e_hi=0)
+             seg_rows=0, segs=0, e_lo=0, e_hi=0, xgroups=0)
+    if p["full"] > 0 and strips % G == 0 and exchange_fits(H, K, strips // G):
+        p["xgroups"] = strips // G
     p["waves"] = p["full"] * strips
     if p["rem"] == 0:
         return p
     p["group"] = min(64 // (p["rem"] + 2), MAX_GROUP)
-    s = p["seg_rows"] = -(-H // strips)
+    if p["xgroups"]:
+        s = p["seg_rows"] = (H // p["xgroups"] + K - 1) // G - (K - 1)
+    else:
+        s = p["seg_rows"] = -(-H // strips)
     p["segs"] = -(-H // s)
     if p["group"] >= 2:
         e_lo = min(-(-K // s), p["segs"])
@@ -84,14 +105,24 @@ def make_plan(W, H, K, strips):
     return p
 
 
-def wave_tile(p, wave, H):
-    """(col_lo, ncols, row_lo, out, seg_lanes, nsegs) of a wave."""
+def wave_tile(p, wave, H, K):
+    """(col_lo, ncols, row_lo, out, seg_lanes, nsegs, exchange) of a wave."""
     whole = p["full"] * p["strips"]
+    if wave < whole and p["xgroups"]:
+        tx = wave // p["strips"]
+        rest = wave - tx * p["strips"]
+        j, wig = rest // G, rest % G
+        g0, g1 = j * H // p["xgroups"], (j + 1) * H // p["xgroups"]
+        hx = (g1 - g0 + K - 1) // G
+        row_lo = g0 + wig * hx
+        exchange = wig < G - 1
+        out = hx if exchange else g1 - row_lo
+        return (tx * OUT_LANES, OUT_LANES, row_lo, out, 64, 1, exchange)
     if wave < whole:
         tx, ty = wave % p["full"], wave // p["full"]
         row_lo = ty * H // p["strips"]
         return (tx * OUT_LANES, OUT_LANES, row_lo,
-                (ty + 1) * H // p["strips"] - row_lo, 64, 1)
+                (ty + 1) * H // p["strips"] - row_lo, 64, 1, False)
     r = wave - whole
     shared = p["e_hi"] - p["e_lo"]
     solo = p["segs"] - shared
@@ -104,83 +135,150 @@ def wave_tile(p, wave, H):
         nsegs = min(p["e_hi"] - seg, p["group"])
     row_lo = seg * p["seg_rows"]
     return (p["full"] * OUT_LANES, p["rem"], row_lo,
-            min(H - row_lo, p["seg_rows"]), seg_lanes, nsegs)
+            min(H - row_lo, p["seg_rows"]), seg_lanes, nsegs, False)
 
 
-def kernel(src, W, H, plan, K, born, sustain, writes):
-    fast = (born & 0x1FF) == 8 and (sustain & 0x1FF) == 12
-    dst = np.full(W * H, 0xDEADBEEF, dtype=np.uint64)
-    nbytes = W * H * 4
-    row_bytes = W * 4
-    lane = np.arange(64)
-    for wave in range(plan["waves"]):
-        col_lo, ncols, row_lo, out, seg_lanes, nsegs = wave_tile(plan, wave, H)
+class Wave:
+    """One wavefront of the kernel: ``prologue`` (the first 2K feeds and
+    the rows handed over) and ``finish`` (after the workgroup barrier)."""
+
+    def __init__(self, src, dst, W, H, plan, K, born, sustain, writes, wave):
+        self.src, self.dst, self.W, self.H, self.K = src, dst, W, H, K
+        self.born, self.sustain, self.writes = born, sustain, writes
+        self.fast = (born & 0x1FF) == 8 and (sustain & 0x1FF) == 12
+        (col_lo, ncols, row_lo, out, seg_lanes, nsegs,
+         self.exchange) = wave_tile(plan, wave, H, K)
         assert 1 <= ncols <= OUT_LANES and out >= 1
+        if self.exchange:
+            assert out >= 2 * K
+        self.wave, self.out, self.row_lo = wave, out, row_lo
+        self.nbytes = W * H * 4
+        self.row_bytes = W * 4
+        lane = np.arange(64)
         seg = lane // seg_lanes
         sl = lane - seg * seg_lanes
-        active = seg < nsegs
-        seg_off = seg * plan["seg_rows"] * row_bytes
+        self.active = seg < nsegs
+        self.nsegs = nsegs
+        seg_off = seg * plan["seg_rows"] * self.row_bytes
         col = wrap(col_lo - 1 + sl, W)
-        lane_off = np.where(active, col * 4 + seg_off, nbytes)
+        self.lane_off = np.where(self.active, col * 4 + seg_off, self.nbytes)
         out_col = col_lo - 1 + sl
-        lane_out = active & (sl >= 1) & (sl <= ncols)
-        out_voff = np.where(lane_out, out_col * 4 + seg_off, nbytes)
-        L = out + 2 * K; row0 = row_lo - K
-        def load_row(i, fed=True):
-            soff = wrap(row0 + i, H) * row_bytes
-            off = lane_off + soff
-            # every load of an active lane is in the field (no reliance on
-            # the range check's treatment of the scalar offset); idle lanes
-            # carry a vector offset past the end and read zero
-            assert np.all(off[active] < nbytes), (wave, i)
-            val = np.zeros(64, dtype=np.uint64)
-            val[active] = src[off[active] // 4]
-            if fed and nsegs > 1:
-                # a shared wave's segments never wrap their rows
-                assert 0 <= row0 + i < H
-            return val
-        def store_row(val, r):
-            keep = r < out
-            voff = out_voff if keep else np.full(64, nbytes)
-            soff = (row_lo + r) * row_bytes if keep else 0
-            for ln in range(64):
-                if voff[ln] < nbytes:   # range check on the vector offset
-                    off = voff[ln] + soff
-                    assert off < nbytes, (wave, r, ln)
-                    dst[off // 4] = val[ln]
-                    writes[off // 4] += 1
-        lev = [[None] * 3 for _ in range(K + 1)]
-        def feed(i, inp, all_levels):
-            assert i < L
-            lev[0][i % 3] = hrow(inp)
-            for l in range(1, K + 1):
-                if all_levels or i >= 2 * l:
-                    j3 = (i - l) % 3
-                    v = life_word(lev[l-1][(j3 + 2) % 3], lev[l-1][j3], lev[l-1][(j3 + 1) % 3], born, sustain, fast)
-                    lev[l][j3] = hrow(v) if l < K else (v, None, None)
-        # prologue: 2K rows, then the first AHEAD rows requested
-        pro = [load_row(r) for r in range(2 * K)]
-        i = 2 * K
-        cur = [load_row(i + r, fed=i + r < L) for r in range(AHEAD)]
+        lane_out = self.active & (sl >= 1) & (sl <= ncols)
+        self.out_voff = np.where(lane_out, out_col * 4 + seg_off, self.nbytes)
+        self.L = out + 2 if self.exchange else out + 2 * K
+        self.row0 = row_lo - K
+        self.lev = [[None] * 3 for _ in range(K + 1)]
+        self.exports = {}
+
+    def load_row(self, i):
+        soff = wrap(self.row0 + i, self.H) * self.row_bytes
+        off = self.lane_off + soff
+        # every load of an active lane is in the field (no reliance on the
+        # range check's treatment of the scalar offset); idle lanes carry a
+        # vector offset past the end and read zero
+        assert np.all(off[self.active] < self.nbytes), (self.wave, i)
+        val = np.zeros(64, dtype=np.uint64)
+        val[self.active] = self.src[off[self.active] // 4]
+        if self.nsegs > 1:
+            # a shared wave's segments never wrap their rows, fed or not
+            assert 0 <= self.row0 + i < self.H
+        return val
+
+    def store_row(self, val, r):
+        keep = r < self.out
+        assert keep
+        voff = self.out_voff
+        soff = (self.row_lo + r) * self.row_bytes
+        for ln in range(64):
+            if voff[ln] < self.nbytes:   # range check on the vector offset
+                off = voff[ln] + soff
+                assert off < self.nbytes, (self.wave, r, ln)
+                self.dst[off // 4] = val[ln]
+                self.writes[off // 4] += 1
+
+    def compute(self, l, i):
+        """Level l's row i - l from level l - 1's rows around it."""
+        K = self.K
+        j3 = (i - l) % 3
+        lv = self.lev[l - 1]
+        v = life_word(lv[(j3 + 2) % 3], lv[j3], lv[(j3 + 1) % 3], self.born,
+                      self.sustain, self.fast)
+        self.lev[l][j3] = hrow(v) if l < K else (v, None, None)
+
+    def feed(self, i, inp, all_levels):
+        assert i < self.L
+        self.lev[0][i % 3] = hrow(inp)
+        for l in range(1, self.K + 1):
+            if all_levels or i >= 2 * l:
+                self.compute(l, i)
+
+    def prologue(self):
+        K = self.K
+        self.pro = [self.load_row(r) for r in range(2 * K)]
+        self.i = 2 * K
+        self.cur = [self.load_row(self.i + r) for r in range(AHEAD)]
         for r in range(2 * K):
-            feed(r, pro[r], False)
+            self.feed(r, self.pro[r], False)
+            if r >= 2:
+                m = r // 2
+                self.exports[(m, r & 1)] = self.lev[m][(r - m) % 3][0].copy()
+
+    def finish(self, successor):
+        K, L = self.K, self.L
+        i, cur = self.i, self.cur
         iters = 0
         while i + AHEAD - 1 < L:
-            nxt = [load_row(i + AHEAD + r, fed=i + AHEAD + r < L)
-                   for r in range(AHEAD)]
+            nxt = [self.load_row(i + AHEAD + r) for r in range(AHEAD)]
             for q in range(AHEAD):
-                feed(i + q, cur[q], True)
-                store_row(lev[K][(i + q - K) % 3][0], i + q - 2 * K)
+                self.feed(i + q, cur[q], True)
+                self.store_row(self.lev[K][(i + q - K) % 3][0], i + q - 2 * K)
             cur = nxt
-            i += AHEAD; iters += 1
+            i += AHEAD
+            iters += 1
             assert iters < 100000
         for r in range(AHEAD):
             if i + r >= L:
                 break
-            feed(i + r, cur[r], True)
-            store_row(lev[K][(i + r - K) % 3][0], i + r - 2 * K)
+            self.feed(i + r, cur[r], True)
+            self.store_row(self.lev[K][(i + r - K) % 3][0], i + r - 2 * K)
+        if not self.exchange or K == 1:
+            return
+        # drain: feed D = 2 .. 2K-1 takes level D // 2's row from the next
+        # strip's exports; the levels above compute, the output is stored
+        assert successor is not None
+        out = self.out
+        for D in range(2, 2 * K):
+            i = out + D
+            mid = D // 2
+            self.lev[mid][(i - mid) % 3] = hrow(successor.exports[(mid, D % 2)])
+            for l in range(mid + 1, K + 1):
+                self.compute(l, i)
+            self.store_row(self.lev[K][(i - K) % 3][0], i - 2 * K)
+
+
+def kernel(src, W, H, plan, K, born, sustain, writes):
+    dst = np.full(W * H, 0xDEADBEEF, dtype=np.uint64)
+    for first in range(0, plan["waves"], G):
+        waves = [Wave(src, dst, W, H, plan, K, born, sustain, writes, w)
+                 for w in range(first, min(first + G, plan["waves"]))]
+        for wv in waves:
+            wv.prologue()
+        # __syncthreads: every wave of the workgroup has exported its rows
+        for g, wv in enumerate(waves):
+            succ = waves[g + 1] if g + 1 < len(waves) else None
+            if wv.exchange:
+                # the successor is the next strip of the same column tile
+                assert succ is not None and succ.row_lo == wv.row_lo + wv.out
+            wv.finish(succ)
     assert not np.any(dst == 0xDEADBEEF), "unwritten words"
     return dst
+
+
+def wave_cost(p, H, K):
+    """``wave_cost``: modelled level rows per level of one wave."""
+    if p["xgroups"]:
+        return (H // p["xgroups"] + K - 1) // G + K + 1.0
+    return H / p["strips"] + 2.0 * K
 
 
 def auto_strips(W, H, depth, simds=1024):
@@ -188,16 +286,29 @@ def auto_strips(W, H, depth, simds=1024):
     issue = (0.0, 0.55, 0.85, 1.0)
     one = make_plan(W, H, depth, 1)
     per_strip = one["full"] + (1.0 / max(one["group"], 1) if one["rem"] else 0)
+    max_groups = H // (2 * depth * G - depth + 1)
     best, best_time = 1, None
     for w in (1, 2, 3):
         target = w * simds
         strips = min(int(target / per_strip) + 1, H)
         while strips > 1 and make_plan(W, H, depth, strips)["waves"] > target:
             strips -= 1
-        waves = make_plan(W, H, depth, strips)["waves"]
-        t = max(1.0, waves / simds) * (H / strips + 2 * depth) / issue[w]
-        if best_time is None or t < best_time:
-            best, best_time = strips, t
+        cands = [strips, 0]
+        xs = min(strips // G * G, max_groups * G)
+        while xs >= G:
+            p = make_plan(W, H, depth, xs)
+            if p["xgroups"] and p["waves"] <= target:
+                cands[1] = xs
+                break
+            xs -= G
+        for c in cands:
+            if c < 1:
+                continue
+            p = make_plan(W, H, depth, c)
+            held = min(w, max(1, -(-p["waves"] // simds)))
+            t = max(1.0, p["waves"] / simds) * wave_cost(p, H, depth) / issue[held]
+            if best_time is None or t < best_time:
+                best, best_time = c, t
     return best
 
 

@@ -189,3 +189,28 @@ def test_shared_remainder_waves_vs_oracle(words, height, depth, strips):
         life.run(depth)
         life.synchronize()
         assert np.array_equal(life.get_cells(), field.cells), (born, sustain)
+
+
+@pytest.mark.parametrize("words,height,depth,strips", [
+    (62, 200, 4, 8), (67, 300, 8, 4), (63, 143, 3, 12), (124, 130, 2, 8),
+    (140, 260, 5, 8), (512, 2048, 8, 16), (512, 1100, 7, 20)])
+def test_exchanging_groups_vs_oracle(words, height, depth, strips):
+    """Plans whose strips hand their last rows over to the next strip of
+    the workgroup through LDS (no halo rows but in each group's last
+    strip), against the oracle over two launches, B3/S23 and a runtime
+    rule."""
+    width = 32 * words
+    rng = np.random.default_rng(words * 104729 + height + depth)
+    cells = rng.integers(0, 2, width * height, dtype=np.uint8)
+    for born, sustain in ((8, 12), rule_masks("B36/S125")):
+        steps = 2 * depth
+        want, field = oracle_hashes(cells, width, height, born, sustain, steps)
+        life = LifeBitsField(width, height, born, sustain, depth=depth,
+                             strips=strips)
+        life.set_cells(cells)
+        life.run(depth)
+        life.synchronize()
+        assert life.crc32() == want[depth - 1], (born, sustain, "launch 1")
+        life.run(depth)
+        life.synchronize()
+        assert np.array_equal(life.get_cells(), field.cells), (born, sustain)

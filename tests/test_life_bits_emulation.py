@@ -93,10 +93,15 @@ def _plan_coverage(words, height, depth, strips):
     plan = E.make_plan(words, height, depth, strips)
     count = np.zeros((height, words), dtype=np.int32)
     for wave in range(plan["waves"]):
-        col_lo, ncols, row_lo, out, seg_lanes, nsegs = E.wave_tile(
-            plan, wave, height)
+        col_lo, ncols, row_lo, out, seg_lanes, nsegs, exchange = E.wave_tile(
+            plan, wave, height, depth)
         assert 1 <= ncols <= E.OUT_LANES and 1 <= out
         assert nsegs * seg_lanes <= 64
+        if exchange:
+            # the next wave of the workgroup holds the next strip's rows
+            assert out >= 2 * depth and (wave + 1) % E.G != 0
+            nxt = E.wave_tile(plan, wave + 1, height, depth)
+            assert nxt[0] == col_lo and nxt[2] == row_lo + out
         last = out + 2 * depth + E.AHEAD - 1     # rows requested: 0 .. last
         for g in range(nsegs):
             lo = row_lo + g * plan["seg_rows"]
@@ -119,3 +124,27 @@ def test_plan_stores_every_word_once(words, height):
         for strips in sorted(s for s in strips_list if s <= height):
             count = _plan_coverage(words, height, depth, strips)
             assert np.all(count == 1), (words, height, depth, strips)
+
+
+@pytest.mark.parametrize("words,height,depth,strips", [
+    (62, 200, 4, 8),     # two groups of 100 rows: strips of 25, last 25
+    (67, 300, 8, 4),     # one group, 5 columns left in halo segments
+    (63, 143, 3, 12),    # groups of 47 and 48 rows
+    (124, 130, 2, 8),    # two whole tiles, strips of 16 and 17 rows
+    (140, 260, 5, 8)])   # two whole tiles + 16 columns, depth 5
+def test_exchanging_groups(words, height, depth, strips):
+    """Strips of a workgroup hand their last rows over to the next strip
+    through LDS instead of computing halo rows: the plan is an exchanging
+    one, every word is stored once and the result is the oracle's, for
+    B3/S23 and a runtime rule."""
+    assert E.make_plan(words, height, depth, strips)["xgroups"] == strips // 4
+    _check(32 * words, height, depth, 8, 12, depth + 1, strips=strips)
+    _check(32 * words, height, depth, 6, 20, depth, strips=strips, seed=3)
+
+
+def test_exchange_needs_strips_of_2k_rows():
+    """Groups whose strips would be shorter than 2K rows run as halo
+    strips."""
+    assert E.make_plan(62, 100, 8, 8)["xgroups"] == 0   # hx = 13 < 16
+    assert E.make_plan(62, 200, 8, 8)["xgroups"] == 2   # hx = 26
+    assert E.make_plan(62, 200, 8, 6)["xgroups"] == 0   # not whole groups

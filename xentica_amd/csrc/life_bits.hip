@@ -27,6 +27,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "xentica_hip.h"
 #include "xentica_kernels.h"
 #include "common.h"
@@ -89,31 +91,99 @@ __device__ __forceinline__ uint32_t level_k_row(const Window<K> &w) {
     return w.lev[K][((I3 - K) % 3 + 3) % 3].raw;
 }
 
+// Drain feed D (2 <= D < 2K) of a tile that hands its last rows over to
+// its successor: the input rows have ended (the tile loads out + 2 of
+// them), level D / 2 takes its row from the successor's first two rows of
+// that level (`imported`), the levels above it compute and the levels
+// below it are done.  I3 is (out + D) % 3, the slot of the feed's row.
+template <int K, bool kFast, int I3, int D>
+__device__ __forceinline__ void drain_feed(Window<K> &w, uint32_t imported,
+                                           const Rule<kFast> &rule) {
+    constexpr int kMid = D / 2;
+    w.lev[kMid][((I3 - kMid) % 3 + 3) % 3] = hrow(imported);
+#pragma unroll
+    for (int l = kMid + 1; l <= K; ++l) {
+        const int j3 = ((I3 - l) % 3 + 3) % 3;
+        const uint32_t v = life_word<kFast>(w.lev[l - 1][(j3 + 2) % 3], w.lev[l - 1][j3],
+                                            w.lev[l - 1][(j3 + 1) % 3], rule);
+        if (l < K)
+            w.lev[l][j3] = hrow(v);
+        else
+            w.lev[l][j3].raw = v;
+    }
+}
+
+// The 2K - 2 drain feeds of a tile whose out % 3 == S, each storing the
+// level-K row it completes (output row out + D - 2K).
+template <int K, bool kFast, int S, typename Store, int... Ds>
+__device__ __forceinline__ void drain_all(Window<K> &w, const uint32_t *imported,
+                                          const Rule<kFast> &rule, Store &store, int out,
+                                          std::integer_sequence<int, Ds...>) {
+    (
+        [&] {
+            constexpr int D = Ds + 2;
+            constexpr int I3 = (S + D) % 3;
+            drain_feed<K, kFast, I3, D>(w, imported[Ds], rule);
+            store(level_k_row<K, I3>(w), out + D - 2 * K);
+        }(),
+        ...);
+}
+
 // Wave plan of one launch.  Waves [0, full * strips) are the whole
-// 62-column tiles over `strips` balanced row strips.  The rem < 62 columns
-// left of the row form the remainder column, cut into `segs` row segments
-// of seg_rows rows (the last one shorter).  Segments [e_lo, e_hi) are
-// interior: their window of rows, including the rows requested ahead,
-// lies inside the field, so they need no row wrap, and they are packed
-// `group` (at most kMaxGroup) to a wave, each in rem + 2 lanes at its own
-// row offset.  The
-// other segments run one to a wave as before.
+// 62-column tiles over `strips` row strips:
+//   * xgroups == 0: balanced strips, each tile computing K halo rows above
+//     and below its own rows (independent of every other wave);
+//   * xgroups > 0: per column tile, xgroups balanced groups of R rows, each
+//     run by one workgroup of kWavesPerGroup waves: its first G - 1 strips
+//     have hx = (R + K - 1) / G rows and hand their last rows over to the
+//     next strip of the group (see the kernel), the last strip keeps the
+//     halo rows and gets R - (G - 1) hx, about K - 1 rows fewer, so that
+//     every wave of the group does the same work.
+// The rem < 62 columns left of the row form the remainder column, cut
+// into `segs` row segments of seg_rows rows (the last one shorter);
+// with xgroups > 0, seg_rows = hx - (K - 1), the rows whose halo work
+// equals an exchanging strip's.  Segments [e_lo, e_hi) are interior: their
+// window of rows, including the rows requested ahead, lies inside the
+// field, so they need no row wrap, and they are packed `group` (at most
+// kMaxGroup) to a wave, each in rem + 2 lanes at its own row offset.  The
+// other segments run one to a wave.
 struct Plan {
-    int full, strips, rem, group, seg_rows, segs, e_lo, e_hi;
+    int full, strips, rem, group, seg_rows, segs, e_lo, e_hi, xgroups;
     long long waves;
 };
 
-// The plan for `strips` row strips of the whole tiles (1 <= strips <= H).
+// Whether `groups` balanced groups of the H rows can run as exchanging
+// workgroups at depth K: every strip but a group's last has at least 2K
+// rows, the last at least one.
+bool exchange_fits(int H, int K, int groups) {
+    if (groups < 1 || groups > H) return false;
+    const int rmin = H / groups;
+    const int rmax = H % groups ? rmin + 1 : rmin;
+    for (int R = rmin; R <= rmax; ++R) {
+        const int hx = (R + K - 1) / kWavesPerGroup;
+        if (hx < 2 * K || R - (kWavesPerGroup - 1) * hx < 1) return false;
+    }
+    return true;
+}
+
+// The plan for `strips` row strips of the whole tiles (1 <= strips <= H);
+// a multiple of kWavesPerGroup whose groups fit runs as exchanging groups.
 Plan make_plan(int W, int H, int K, int strips) {
     Plan p{};
     p.full = W / kOutLanes;
     p.rem = W - p.full * kOutLanes;
     p.strips = strips;
+    if (p.full > 0 && strips % kWavesPerGroup == 0 &&
+        exchange_fits(H, K, strips / kWavesPerGroup))
+        p.xgroups = strips / kWavesPerGroup;
     p.waves = static_cast<long long>(p.full) * strips;
     if (p.rem == 0) return p;
     p.group = kWave / (p.rem + 2);
     if (p.group > kMaxGroup) p.group = kMaxGroup;
-    p.seg_rows = (H + strips - 1) / strips;
+    if (p.xgroups > 0)
+        p.seg_rows = (H / p.xgroups + K - 1) / kWavesPerGroup - (K - 1);
+    else
+        p.seg_rows = (H + strips - 1) / strips;
     p.segs = (H + p.seg_rows - 1) / p.seg_rows;
     if (p.group >= 2) {
         // interior: j * seg_rows >= K and (j + 1) * seg_rows + K + kAhead <= H
@@ -131,23 +201,50 @@ Plan make_plan(int W, int H, int K, int strips) {
 // output word columns [col_lo, col_lo + ncols) and output rows
 // [row_lo, row_lo + out) (see Plan; a shared remainder wave owns nsegs
 // such tiles, segment g at rows row_lo + g * seg_rows).  It streams the
-// L = out + 2K rows around them through a K-level row pipeline: when input
-// row i arrives, level l computes row i - l from rows i-l-1 .. i-l+1 of
-// level l-1, so level l holds valid rows [l, L-1-l] and only three rows per
-// level are live.
+// rows from row_lo - K on through a K-level row pipeline: when input row i
+// arrives, level l computes row i - l from rows i-l-1 .. i-l+1 of level
+// l-1, so only three rows per level are live.
+//   * A halo tile loads L = out + 2K rows, level l computing rows
+//     [l, L - l): the rows past out + l - 1 only feed this tile's own last
+//     output rows, and the next strip computes them as well.
+//   * An exchanging tile (a workgroup's strips but the last) loads out + 2
+//     rows and computes only its rows [l, out + l) of each level.  Every
+//     tile writes its first two rows of each level 1 .. K-1 (rows l and
+//     l + 1, complete after the first 2K input rows) to LDS; after a
+//     barrier, the exchanging tile's last 2K - 2 feeds take the rows
+//     out + l and out + l + 1 of level l from the next strip's, which are
+//     the same rows of the field.
 template <int K, bool kFast>
 __global__ __launch_bounds__(kWave * kWavesPerGroup)
 __attribute__((amdgpu_waves_per_eu(kMinWavesPerSimd)))
 void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
                       int W, int H, Plan plan, uint32_t born, uint32_t sustain) {
+    constexpr int kLevels = K > 1 ? K - 1 : 1;  // levels handed over
+    __shared__ uint32_t xch[kWavesPerGroup][kLevels][2][kWave];
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform tile coordinates live in scalar registers
-    const int wave = __builtin_amdgcn_readfirstlane(
-        static_cast<int>(blockIdx.x) * kWavesPerGroup + (static_cast<int>(threadIdx.x) >> 6));
-    if (wave >= plan.waves) return;
+    const int wig = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int wave = static_cast<int>(blockIdx.x) * kWavesPerGroup + wig;
+    // waves past the plan still take part in the workgroup's barrier
+    const bool live = wave < plan.waves;
     const int whole = plan.full * plan.strips;
-    int col_lo, ncols, row_lo, out, seg_lanes = kWave, nsegs = 1;
-    if (wave < whole) {
+    int col_lo = 0, ncols = 1, row_lo = 0, out = 1, seg_lanes = kWave, nsegs = 1;
+    bool exchange = false;
+    if (!live) {
+        nsegs = 0;
+    } else if (wave < whole && plan.xgroups > 0) {
+        const int tx = wave / plan.strips;
+        const int rest = wave - tx * plan.strips;
+        const int j = rest / kWavesPerGroup;  // the workgroup's row group
+        const int g0 = static_cast<int>((static_cast<long long>(j) * H) / plan.xgroups);
+        const int g1 = static_cast<int>((static_cast<long long>(j + 1) * H) / plan.xgroups);
+        const int hx = (g1 - g0 + K - 1) / kWavesPerGroup;
+        col_lo = tx * kOutLanes;
+        ncols = kOutLanes;
+        row_lo = g0 + wig * hx;
+        exchange = wig < kWavesPerGroup - 1;
+        out = exchange ? hx : g1 - row_lo;
+    } else if (wave < whole) {
         const int tx = wave % plan.full;
         const int ty = wave / plan.full;
         col_lo = tx * kOutLanes;
@@ -172,7 +269,7 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
         row_lo = seg * plan.seg_rows;
         out = (H - row_lo < plan.seg_rows ? H - row_lo : plan.seg_rows);
     }
-    const int L = out + 2 * K;
+    const int L = exchange ? out + 2 : out + 2 * K;  // input rows fed
     const int row0 = row_lo - K;
 
     // Buffer resources: scalar base and row offsets, one per-lane column
@@ -222,22 +319,33 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
     // rows and the first nine are all requested before the first feed.
     // Rows past L-1 are wrapped rows: in bounds, loaded and never fed.
     constexpr int P0 = (2 * K) % 3, P1 = (2 * K + 1) % 3, P2 = (2 * K + 2) % 3;
-    uint32_t pro[2 * K];
-#pragma unroll
-    for (int r = 0; r < 2 * K; ++r) pro[r] = load_row(r);
     int i = 2 * K;
     uint32_t cur[9], nxt[9];
+    uint32_t *const mine = &xch[wig][0][0][lane];
+    if (live) {
+        uint32_t pro[2 * K];
 #pragma unroll
-    for (int r = 0; r < 9; ++r) cur[r] = load_row(i + r);
-    // prologue: rows 0 .. 2K-1 fill the pipeline (no output yet)
+        for (int r = 0; r < 2 * K; ++r) pro[r] = load_row(r);
 #pragma unroll
-    for (int r = 0; r < 2 * K; ++r) {
-        switch (r % 3) {
-            case 0: feed<K, kFast, 0, false>(w, pro[r], r, rule); break;
-            case 1: feed<K, kFast, 1, false>(w, pro[r], r, rule); break;
-            default: feed<K, kFast, 2, false>(w, pro[r], r, rule); break;
+        for (int r = 0; r < 9; ++r) cur[r] = load_row(i + r);
+        // prologue: rows 0 .. 2K-1 fill the pipeline (no output yet); feed r
+        // completes row r - l of each level l <= r / 2, the first two rows of
+        // level r / 2 at r = 2l and 2l + 1: those go to LDS
+#pragma unroll
+        for (int r = 0; r < 2 * K; ++r) {
+            switch (r % 3) {
+                case 0: feed<K, kFast, 0, false>(w, pro[r], r, rule); break;
+                case 1: feed<K, kFast, 1, false>(w, pro[r], r, rule); break;
+                default: feed<K, kFast, 2, false>(w, pro[r], r, rule); break;
+            }
+            if (r >= 2) {
+                const int m = r / 2;
+                mine[((m - 1) * 2 + (r & 1)) * kWave] = w.lev[m][((r - m) % 3 + 3) % 3].raw;
+            }
         }
     }
+    __syncthreads();
+    if (!live) return;
     // rows row .. row+2 (slots P0, P1, P2 by row mod 3), each output row
     // r = row - 2K stored as its level-K row leaves the pipeline
     auto feed3 = [&](uint32_t x0, uint32_t x1, uint32_t x2, int row) {
@@ -257,6 +365,10 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
     for (; i + 8 < L; i += 9) {
 #pragma unroll
         for (int r = 0; r < 9; ++r) nxt[r] = load_row(i + 9 + r);
+        // keep the nine loads at the top of the iteration: the scheduler
+        // otherwise sinks them towards their use, and the hand-over at the
+        // loop head then waits on loads issued late in the iteration
+        __builtin_amdgcn_sched_barrier(0);
         feed3(cur[0], cur[1], cur[2], i);
         feed3(cur[3], cur[4], cur[5], i + 3);
         feed3(cur[6], cur[7], cur[8], i + 6);
@@ -280,6 +392,21 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
                 feed<K, kFast, 2, true>(w, cur[r], i + r, rule);
                 store_row(level_k_row<K, 2>(w), i + r - 2 * K);
                 break;
+        }
+    }
+    if constexpr (K > 1) {
+        if (!exchange) return;
+        // drain: the next strip's first two rows of each level 1 .. K-1, in
+        // the order the feeds D = 2 .. 2K-1 take them (level D / 2, row D % 2)
+        uint32_t imported[2 * K - 2];
+        const uint32_t *const next = &xch[wig + 1][0][0][lane];
+#pragma unroll
+        for (int d = 0; d < 2 * K - 2; ++d) imported[d] = next[d * kWave];
+        using Feeds = std::make_integer_sequence<int, 2 * K - 2>;
+        switch (out % 3) {
+            case 0: drain_all<K, kFast, 0>(w, imported, rule, store_row, out, Feeds{}); break;
+            case 1: drain_all<K, kFast, 1>(w, imported, rule, store_row, out, Feeds{}); break;
+            default: drain_all<K, kFast, 2>(w, imported, rule, store_row, out, Feeds{}); break;
         }
     }
 }
@@ -310,21 +437,32 @@ int launch_any_k(int k, const uint32_t *src, uint32_t *dst, int W, int H, const 
     }
 }
 
+// Modelled cost of one wave of a plan, in level rows per level: a halo
+// strip of S rows streams S + 2K rows; an exchanging group's waves do the
+// work of hx - (K - 1) output rows of a halo strip (the last strip of a
+// group and the remainder segments are sized to match).
+double wave_cost(const Plan &p, int H, int K) {
+    if (p.xgroups > 0)
+        return static_cast<double>((H / p.xgroups + K - 1) / kWavesPerGroup) + K + 1.0;
+    return static_cast<double>(H) / static_cast<double>(p.strips) + 2.0 * K;
+}
+
 // Row strips per launch when the caller passes 0.  A launch is bound by
-// VALU issue, and a strip of S output rows costs (S + 2K) level rows per
-// level, so shorter strips add halo work while more waves per SIMD hide
-// more latency.  For w = 1, 2, 3 waves per SIMD of the device (the most
-// strips whose plan has at most w waves per SIMD, so that no CU receives
-// one workgroup more than the others, and at most one strip per row) the
-// modelled time is
-//   max(1, waves / SIMDs) * (S + 2K) / issue(w),
+// VALU issue; shorter strips add halo work while more waves per SIMD hide
+// more latency.  For w = 1, 2, 3 waves per SIMD of the device it takes
+// the most strips whose plan has at most w waves per SIMD (so that no CU
+// receives one workgroup more than the others, at most one strip per
+// row), once over all strip counts and once over the counts that run as
+// exchanging groups, and models the time as
+//   max(1, waves / SIMDs) * wave_cost / issue(min(w, ceil(waves / SIMDs))),
 // with issue(w) the share of a SIMD's VALU slots that w waves keep busy,
 // measured on MI355X at depth 8: 0.55, 0.85 and 1.  The smallest wins.
-// Measured optima of the model with whole-row tiles (no shared remainder
-// waves): 16384^2 three waves per SIMD (4.6 us per step against 5.6 at 1.5
-// waves); 4096^2 and 2048^2 one wave per SIMD (1.27 and 0.83 us per step,
-// against 1.89 and 1.78 with strips of at least 4K rows); 512^2 and
-// 640x360 a strip per row (0.74 and 0.76 us against 1.78 and 1.80).
+// Measured optima of the model with balanced halo strips: 16384^2 three
+// waves per SIMD (4.6 us per step against 5.6 at 1.5 waves); 4096^2 and
+// 2048^2 one wave per SIMD (1.27 and 0.83 us per step, against 1.89 and
+// 1.78 with strips of at least 4K rows); 512^2 and 640x360 a strip per row
+// (0.74 and 0.76 us against 1.78 and 1.80).  Results are kept per field
+// shape and depth.
 int auto_strips(int W, int H, int depth) {
     static int simds = 0;
     if (!simds) {
@@ -335,29 +473,56 @@ int auto_strips(int W, int H, int depth) {
         else
             simds = 1024;
     }
+    thread_local int last_w = -1, last_h = -1, last_k = -1, last_s = 1;
+    if (W == last_w && H == last_h && depth == last_k) return last_s;
     constexpr double kIssue[4] = {0.0, 0.55, 0.85, 1.0};
     const Plan one = make_plan(W, H, depth, 1);
     // waves per strip, for the first guess
     const double per_strip = one.full + (one.rem ? 1.0 / (one.group > 1 ? one.group : 1) : 0.0);
+    // the most groups whose strips can have 2K rows
+    const int max_groups = H / (2 * depth * kWavesPerGroup - depth + 1);
     long long best = 1;
     double best_time = 0.0;
+    bool first = true;
     for (int w = 1; w <= 3; ++w) {
         const long long target = static_cast<long long>(w) * simds;
         long long strips = static_cast<long long>(target / per_strip) + 1;
         if (strips > H) strips = H;
         while (strips > 1 && make_plan(W, H, depth, static_cast<int>(strips)).waves > target)
             --strips;
-        const long long waves = make_plan(W, H, depth, static_cast<int>(strips)).waves;
-        const double rows = static_cast<double>(H) / static_cast<double>(strips);
-        double per_simd = static_cast<double>(waves) / simds;
-        if (per_simd < 1.0) per_simd = 1.0;
-        const double time = per_simd * (rows + 2.0 * depth) / kIssue[w];
-        if (w == 1 || time < best_time) {
-            best = strips;
-            best_time = time;
+        long long cands[2] = {strips, 0};
+        long long xs = strips / kWavesPerGroup * kWavesPerGroup;
+        if (xs > static_cast<long long>(max_groups) * kWavesPerGroup)
+            xs = static_cast<long long>(max_groups) * kWavesPerGroup;
+        for (; xs >= kWavesPerGroup; xs -= kWavesPerGroup) {
+            const Plan p = make_plan(W, H, depth, static_cast<int>(xs));
+            if (p.xgroups > 0 && p.waves <= target) {
+                cands[1] = xs;
+                break;
+            }
+        }
+        for (long long c : cands) {
+            if (c < 1) continue;
+            const Plan p = make_plan(W, H, depth, static_cast<int>(c));
+            double per_simd = static_cast<double>(p.waves) / simds;
+            if (per_simd < 1.0) per_simd = 1.0;
+            // the issue share of the waves a SIMD actually holds
+            long long held = (p.waves + simds - 1) / simds;
+            if (held > w) held = w;
+            if (held < 1) held = 1;
+            const double time = per_simd * wave_cost(p, H, depth) / kIssue[held];
+            if (first || time < best_time) {
+                best = c;
+                best_time = time;
+                first = false;
+            }
         }
     }
-    return static_cast<int>(best);
+    last_w = W;
+    last_h = H;
+    last_k = depth;
+    last_s = static_cast<int>(best);
+    return last_s;
 }
 
 }  // namespace
