@@ -63,6 +63,7 @@ OUT_LANES = 62
 AHEAD = 9
 MAX_GROUP = 4
 G = 4           # waves per workgroup
+MIN_EXCHANGE_DEPTH = 4
 
 
 def exchange_fits(H, K, groups):
@@ -85,7 +86,8 @@ def make_plan(W, H, K, strips):
     segments, interior ones packed ``group`` to a wave."""
     p = dict(full=W // OUT_LANES, rem=W % OUT_LANES, strips=strips, group=0,
              seg_rows=0, segs=0, e_lo=0, e_hi=0, xgroups=0)
-    if p["full"] > 0 and strips % G == 0 and exchange_fits(H, K, strips // G):
+    if p["full"] > 0 and K >= MIN_EXCHANGE_DEPTH and strips % G == 0 and \
+            exchange_fits(H, K, strips // G):
         p["xgroups"] = strips // G
     p["waves"] = p["full"] * strips
     if p["rem"] == 0:

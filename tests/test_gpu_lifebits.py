@@ -192,7 +192,7 @@ def test_shared_remainder_waves_vs_oracle(words, height, depth, strips):
 
 
 @pytest.mark.parametrize("words,height,depth,strips", [
-    (62, 200, 4, 8), (67, 300, 8, 4), (63, 143, 3, 12), (124, 130, 2, 8),
+    (62, 200, 4, 8), (67, 300, 8, 4), (63, 143, 4, 12), (124, 170, 4, 8),
     (140, 260, 5, 8), (512, 2048, 8, 16), (512, 1100, 7, 20)])
 def test_exchanging_groups_vs_oracle(words, height, depth, strips):
     """Plans whose strips hand their last rows over to the next strip of

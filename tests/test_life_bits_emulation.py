@@ -129,8 +129,8 @@ def test_plan_stores_every_word_once(words, height):
 @pytest.mark.parametrize("words,height,depth,strips", [
     (62, 200, 4, 8),     # two groups of 100 rows: strips of 25, last 25
     (67, 300, 8, 4),     # one group, 5 columns left in halo segments
-    (63, 143, 3, 12),    # groups of 47 and 48 rows
-    (124, 130, 2, 8),    # two whole tiles, strips of 16 and 17 rows
+    (63, 143, 4, 12),    # groups of 47 and 48 rows
+    (124, 170, 4, 8),    # two whole tiles, strips of 22 rows (the last 19)
     (140, 260, 5, 8)])   # two whole tiles + 16 columns, depth 5
 def test_exchanging_groups(words, height, depth, strips):
     """Strips of a workgroup hand their last rows over to the next strip
@@ -148,3 +148,4 @@ def test_exchange_needs_strips_of_2k_rows():
     assert E.make_plan(62, 100, 8, 8)["xgroups"] == 0   # hx = 13 < 16
     assert E.make_plan(62, 200, 8, 8)["xgroups"] == 2   # hx = 26
     assert E.make_plan(62, 200, 8, 6)["xgroups"] == 0   # not whole groups
+    assert E.make_plan(62, 200, 3, 8)["xgroups"] == 0   # depth below 4

@@ -48,6 +48,11 @@ constexpr int kMaxGroup = 4;
 // nine rows of the next iteration, so a wave reads up to row L + 8 of its
 // window of L rows
 constexpr int kAhead = 9;
+// Depth from which whole tiles run as exchanging groups: the halo rows
+// they save grow as K (K - 1), and at depths 1 and 2 the plain strips
+// measured faster (16384^2: 15.2 and 15.9 us per launch against 17.6 and
+// 18.9)
+constexpr int kMinExchangeDepth = 4;
 // dword 3 of a raw buffer resource on gfx9-family (CDNA) parts
 constexpr int kRsrcFlags = 0x00020000;
 // keep several waves per SIMD resident (caps VGPRs at 512 / this)
@@ -167,13 +172,14 @@ bool exchange_fits(int H, int K, int groups) {
 }
 
 // The plan for `strips` row strips of the whole tiles (1 <= strips <= H);
-// a multiple of kWavesPerGroup whose groups fit runs as exchanging groups.
+// at depths of at least kMinExchangeDepth, a multiple of kWavesPerGroup
+// whose groups fit runs as exchanging groups.
 Plan make_plan(int W, int H, int K, int strips) {
     Plan p{};
     p.full = W / kOutLanes;
     p.rem = W - p.full * kOutLanes;
     p.strips = strips;
-    if (p.full > 0 && strips % kWavesPerGroup == 0 &&
+    if (p.full > 0 && K >= kMinExchangeDepth && strips % kWavesPerGroup == 0 &&
         exchange_fits(H, K, strips / kWavesPerGroup))
         p.xgroups = strips / kWavesPerGroup;
     p.waves = static_cast<long long>(p.full) * strips;
