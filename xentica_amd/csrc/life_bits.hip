@@ -300,9 +300,17 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
     const int out_row0 = row_lo;
     const int out_rows = out;
 
-    auto load_row = [&](int i) -> uint32_t {
-        return __builtin_amdgcn_raw_buffer_load_b32(in_rsrc, lane_off,
-                                                    wrap(row0 + i, H) * row_bytes, 0);
+    // Input rows are requested strictly in order (row0, row0 + 1, ... on
+    // the torus), so the scalar byte offset of the next one advances by a
+    // row and wraps to 0 at the field's end: three scalar instructions a
+    // row instead of a signed remainder.
+    const int field_bytes = H * row_bytes;
+    int next_off = wrap(row0, H) * row_bytes;
+    auto load_next = [&]() -> uint32_t {
+        const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(in_rsrc, lane_off, next_off, 0);
+        next_off += row_bytes;
+        next_off = next_off == field_bytes ? 0 : next_off;
+        return v;
     };
     // output row r = i - 2K of the tile, fed at input row i
     auto store_row = [&](uint32_t val, int r) {
@@ -331,9 +339,9 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
     if (live) {
         uint32_t pro[2 * K];
 #pragma unroll
-        for (int r = 0; r < 2 * K; ++r) pro[r] = load_row(r);
+        for (int r = 0; r < 2 * K; ++r) pro[r] = load_next();
 #pragma unroll
-        for (int r = 0; r < 9; ++r) cur[r] = load_row(i + r);
+        for (int r = 0; r < 9; ++r) cur[r] = load_next();
         // prologue: rows 0 .. 2K-1 fill the pipeline (no output yet); feed r
         // completes row r - l of each level l <= r / 2, the first two rows of
         // level r / 2 at r = 2l and 2l + 1: those go to LDS
@@ -370,7 +378,7 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
     __builtin_amdgcn_s_waitcnt(kVmcntZero);
     for (; i + 8 < L; i += 9) {
 #pragma unroll
-        for (int r = 0; r < 9; ++r) nxt[r] = load_row(i + 9 + r);
+        for (int r = 0; r < 9; ++r) nxt[r] = load_next();  // rows i + 9 .. i + 17
         // keep the nine loads at the top of the iteration: the scheduler
         // otherwise sinks them towards their use, and the hand-over at the
         // loop head then waits on loads issued late in the iteration
