@@ -570,14 +570,14 @@ def energy_cpu_baseline(steps=10):
     return host_baseline(
         EnergyLife(cpu_exp), steps,
         "EnergyLife 4096x4096 word 42, %d steps after 1 warm-up step: the "
-        "DSL's generated emit/absorb (8 buffer planes of 128-bit words, "
+        "DSL's generated emit/absorb (8 buffer planes of 64-bit words, "
         "colours) compiled for the host, g++ -O3 -fopenmp" % steps)
 
 
 def energy_life(steps=5000, plain_steps=20, cpu=True, tiles=None,
                 depths=None):
     """
-    EnergyLife on a 4096x4096 torus: 128-bit words (energy 16 bits, genome
+    EnergyLife on a 4096x4096 torus: 64-bit words (energy 8 bits, genome
     18, mode 1, xorshift32 32), word 42, ``steps`` steps by the fused
     engine in one graph-captured run; total energy checked before and
     after; the plain emit/absorb launch loop timed over ``plain_steps``.
@@ -614,9 +614,9 @@ def energy_life(steps=5000, plain_steps=20, cpu=True, tiles=None,
             plain.step()
         hip.device().synchronize()
         plain_value = n * plain_steps / (time.perf_counter() - t0)
-    bpc = 32.0
+    bpc = 16.0
     return {
-        "workload": "EnergyLife 4096x4096 torus, 128-bit words (energy 16, "
+        "workload": "EnergyLife 4096x4096 torus, 64-bit words (energy 8, "
                     "genome 18, mode 1, xorshift32), word 42, %d steps"
                     % steps,
         "value": value, "unit": "cell-updates/s", "ms_per_step": ms / steps,

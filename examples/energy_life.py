@@ -7,9 +7,11 @@ a genome, births that inherit a neighbour's genome, a per-cell RNG), and
 adds what ``EvoLife`` does not have: exact energy conservation and a
 32-bit xorshift generator.
 
-State of a cell (67 bits, so the container is one 128-bit word):
+State of a cell (59 bits, so the container is one 64-bit word):
 
-* ``energy`` -- 16 bits, seeded uniform 0-255;
+* ``energy`` -- 8 bits, seeded uniform 0-255 (a cell's new energy lies
+  between the smallest and the largest energy of its neighbourhood, see
+  below, so energies never leave the seeded range);
 * ``genome`` -- 18 bits, birth counts in bits 0-8 and survival counts in
   bits 9-17 (the encoding of ``LifeLike.golly2int``);
 * ``mode`` -- 1 bit, the cell is alive;
@@ -24,7 +26,7 @@ One step:
   of an edge compute the same flux, so the total is conserved exactly,
   and since every flux is at most 1/8 of the difference a cell's new
   energy stays between the minimum and maximum of its neighbourhood
-  (16 bits never overflow);
+  (8 bits never overflow for energies seeded in 0-255);
 * a live cell survives if its genome's survival bit for its live-
   neighbour count is set and its new energy is at least
   ``sustain_energy``; a dead cell is born if the genome of its richest
@@ -43,7 +45,7 @@ from examples.base import RegularCA
 class EnergyLife(RegularCA):
     """Conserved energy, evolving Life-like genomes, xorshift32 per cell."""
 
-    energy = core.IntegerProperty(max_val=2 ** 16 - 1)
+    energy = core.IntegerProperty(max_val=2 ** 8 - 1)
     genome = core.IntegerProperty(max_val=2 ** 18 - 1)
     mode = core.IntegerProperty(max_val=1)
     rng = core.XorShiftProperty()
@@ -111,6 +113,14 @@ class EnergyLife(RegularCA):
         return (red, green, blue)
 
 
+class EnergyLifeWide(EnergyLife):
+    """The same model with a 16-bit energy field: 67 bits of state, so one
+    128-bit word per cell (the engines' wide-word path; energies above
+    255 are representable, the dynamics are unchanged)."""
+
+    energy = core.IntegerProperty(max_val=2 ** 16 - 1)
+
+
 class EnergyLifeExperiment(core.Experiment):
     """4096x4096 torus, word 42, uniform energies, genomes and modes."""
 
@@ -150,6 +160,9 @@ def field_values(model, name):
     import numpy as np
     shift, width = field_position(model, name)
     words = model.gpu.arrays.cells.get()[:model.cells_num]
+    if words.dtype.names is None:
+        val = words.astype(np.uint64) >> np.uint64(shift)
+        return (val & np.uint64((1 << width) - 1)).astype(np.int64)
     lo = words['lo'].astype(np.uint64)
     hi = words['hi'].astype(np.uint64)
     if shift >= 64:

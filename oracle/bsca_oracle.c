@@ -27,9 +27,10 @@
  *   oracle_shifting_sands ShiftingSands (examples/shifting_sands.py:15-47)
  *   oracle_noisetv       NoiseTV (examples/noisetv.py:10-28)
  *   oracle_energy_life   EnergyLife (no reference counterpart: the
- *                        project's EvoLife-style model on a 128-bit word,
- *                        written in the reference's DSL; the rule is
- *                        restated here from its definition)
+ *                        project's EvoLife-style model on a 64-bit word,
+ *                        and its 128-bit variant, written in the
+ *                        reference's DSL; the rule is restated here from
+ *                        its definition)
  *   oracle_xorshift32_period  period of the xorshift32 map from a state
  *   oracle_crc32         CRC-32 of binascii.crc32 (tests/core/test_base.py:39-41)
  *
@@ -287,29 +288,63 @@ int oracle_noisetv(uint16_t *fld, int32_t *col, int64_t n,
 }
 
 /*
- * EnergyLife on a 128-bit word: energy (16 bits at 0), genome (18 bits at
- * 16: birth mask bits 0-8, survival mask bits 9-17), mode (1 bit at 34),
- * xorshift32 state (32 bits at 35).  Emit publishes energy, genome and
- * mode (generator field 0) in every buffer slot.  Absorb, for each
- * neighbour j with energy ej and mode mj: the energy moves by
- * sign(ej - e) * (|ej - e| >> (3 + m + mj)); count = number of live
- * neighbours; the richest live neighbour (first on ties, energy > 0) is
- * the parent.  The generator advances once (x ^= x << 13; x ^= x >> 17;
- * x ^= x << 5).  A live cell stays alive iff its survival bit for count
- * is set and its new energy >= sustain; a dead cell becomes alive iff
- * the parent's birth bit for count is set and its new energy >= birth;
- * a newborn takes the parent's genome, with bit ((r >> 16) % 18) flipped
- * when (r & 65535) < mutation.  Colour: MovingAverage of (mode * 255,
- * energy, (genome >> 10) & 255) of the new state.
+ * EnergyLife (examples/energy_life.py) on one word per cell, in either of
+ * the model's two layouts (the DSL packs fields in declaration order from
+ * bit 0):
+ *   narrow (64-bit word): energy 8 bits at 0, genome 18 at 8, mode 1 at
+ *     26, xorshift32 state 32 at 27;
+ *   wide (128-bit word):  energy 16 bits at 0, genome 18 at 16, mode 1 at
+ *     34, xorshift32 state 32 at 35.
+ * genome: birth mask bits 0-8, survival mask bits 9-17.  Emit publishes
+ * energy, genome and mode (generator field 0) in every buffer slot.
+ * Absorb, for each neighbour j with energy ej and mode mj: the energy
+ * moves by sign(ej - e) * (|ej - e| >> (3 + m + mj)); count = number of
+ * live neighbours; the richest live neighbour (first on ties, energy > 0)
+ * is the parent.  The generator advances once (x ^= x << 13;
+ * x ^= x >> 17; x ^= x << 5).  A live cell stays alive iff its survival
+ * bit for count is set and its new energy >= sustain; a dead cell
+ * becomes alive iff the parent's birth bit for count is set and its new
+ * energy >= birth; a newborn takes the parent's genome, with bit
+ * ((r >> 16) % 18) flipped when (r & 65535) < mutation.  Colour:
+ * MovingAverage of (mode * 255, energy, (genome >> 10) & 255) of the new
+ * state.  New energies stay between the neighbourhood's minimum and
+ * maximum, so the narrow layout's 8 bits hold any field seeded in 0-255.
  */
 typedef unsigned __int128 u128;
 
-static inline u128 el_pack(uint32_t e, uint32_t g, uint32_t m, uint32_t r) {
-    return (u128)(e & 65535u) | ((u128)(g & 262143u) << 16) |
-           ((u128)(m & 1u) << 34) | ((u128)r << 35);
+typedef struct {
+    int wide;       /* 16-byte words (else 8-byte) */
+    int e_bits;     /* energy width; genome, mode and rng follow it */
+} el_layout_t;
+
+static inline u128 el_load(const void *fld, int64_t i, const el_layout_t *lay) {
+    return lay->wide ? ((const u128 *)fld)[i] : (u128)((const uint64_t *)fld)[i];
 }
 
-int oracle_energy_life(u128 *fld, int32_t *col, int64_t w0, int64_t w1,
+static inline void el_store(void *fld, int64_t i, u128 w, const el_layout_t *lay) {
+    if (lay->wide)
+        ((u128 *)fld)[i] = w;
+    else
+        ((uint64_t *)fld)[i] = (uint64_t)w;
+}
+
+static inline u128 el_pack(uint32_t e, uint32_t g, uint32_t m, uint32_t r,
+                           const el_layout_t *lay) {
+    const int eb = lay->e_bits;
+    return (u128)(e & ((1u << eb) - 1u)) | ((u128)(g & 262143u) << eb) |
+           ((u128)(m & 1u) << (eb + 18)) | ((u128)r << (eb + 19));
+}
+
+static inline void el_unpack(u128 w, const el_layout_t *lay, uint32_t *e, uint32_t *g,
+                             uint32_t *m, uint32_t *r) {
+    const int eb = lay->e_bits;
+    *e = (uint32_t)(w & ((1u << eb) - 1u));
+    *g = (uint32_t)((w >> eb) & 262143u);
+    *m = (uint32_t)((w >> (eb + 18)) & 1u);
+    *r = (uint32_t)(w >> (eb + 19));
+}
+
+int oracle_energy_life(void *fld, int wide, int32_t *col, int64_t w0, int64_t w1,
                        const int8_t *delta, const int32_t *rev,
                        uint32_t sustain, uint32_t birth, uint32_t mutation,
                        int fade_in, int fade_out, int smooth, int steps,
@@ -318,28 +353,25 @@ int oracle_energy_life(u128 *fld, int32_t *col, int64_t w0, int64_t w1,
     int64_t sizes[2] = {w0, w1};
     if (setup(&lat, 2, sizes, 8, delta, rev, 0, 0)) return 1;
     set_threads(threads);
+    const el_layout_t lay = {wide, wide ? 16 : 8};
     const int64_t n = lat.n;
     for (int s = 0; s < steps; ++s) {
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; ++i) {
-            u128 w = fld[i];
-            u128 b = el_pack((uint32_t)(w & 65535u), (uint32_t)((w >> 16) & 262143u),
-                             (uint32_t)((w >> 34) & 1u), 0);
-            for (int q = 0; q < 8; ++q) fld[i + n * (rev[q] + 1)] = b;
+            uint32_t e, g, m, r;
+            el_unpack(el_load(fld, i, &lay), &lay, &e, &g, &m, &r);
+            const u128 b = el_pack(e, g, m, 0, &lay);
+            for (int q = 0; q < 8; ++q) el_store(fld, i + n * (rev[q] + 1), b, &lay);
         }
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < n; ++i) {
-            u128 w = fld[i];
-            uint32_t e = (uint32_t)(w & 65535u);
-            uint32_t g = (uint32_t)((w >> 16) & 262143u);
-            uint32_t m = (uint32_t)((w >> 34) & 1u);
-            uint32_t r = (uint32_t)(w >> 35);
+            uint32_t e, g, m, r;
+            el_unpack(el_load(fld, i, &lay), &lay, &e, &g, &m, &r);
             uint32_t ne = e, count = 0, best_e = 0, best_g = 0;
             for (int j = 0; j < 8; ++j) {
-                u128 v = fld[neighbor_index(&lat, i, j) + n * (j + 1)];
-                uint32_t ej = (uint32_t)(v & 65535u);
-                uint32_t gj = (uint32_t)((v >> 16) & 262143u);
-                uint32_t mj = (uint32_t)((v >> 34) & 1u);
+                uint32_t ej, gj, mj, rj;
+                el_unpack(el_load(fld, neighbor_index(&lat, i, j) + n * (j + 1), &lay), &lay,
+                          &ej, &gj, &mj, &rj);
                 uint32_t sh = 3 + m + mj;
                 if (ej > e) ne += (ej - e) >> sh;
                 if (e > ej) ne -= (e - ej) >> sh;
@@ -362,13 +394,15 @@ int oracle_energy_life(u128 *fld, int32_t *col, int64_t w0, int64_t w1,
                 if ((r & 65535u) < mutation) g ^= 1u << ((r >> 16) % 18u);
             }
             if (col) {
+                const uint32_t emask = (1u << lay.e_bits) - 1u;
                 col[3 * i + 0] = fade((int32_t)(nm * 255), col[3 * i + 0], fade_in, fade_out, smooth);
-                col[3 * i + 1] = fade((int32_t)(ne & 65535u), col[3 * i + 1], fade_in, fade_out, smooth);
+                col[3 * i + 1] = fade((int32_t)(ne & emask), col[3 * i + 1], fade_in, fade_out, smooth);
                 col[3 * i + 2] = fade((int32_t)((g >> 10) & 255u), col[3 * i + 2], fade_in, fade_out, smooth);
             }
-            fld[i] = el_pack(ne, g, nm, r);
+            el_store(fld, i, el_pack(ne, g, nm, r, &lay), &lay);
         }
-        if (hashes) hashes[s] = oracle_crc32(0, (const uint8_t *)fld, n * 16);
+        if (hashes)
+            hashes[s] = oracle_crc32(0, (const uint8_t *)fld, n * (wide ? 16 : 8));
     }
     return 0;
 }

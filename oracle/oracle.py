@@ -52,7 +52,8 @@ __all__ = [
     'library', 'local_random', 'moore_deltas', 'von_neumann_deltas',
     'moore_rev', 'von_neumann_rev', 'big_bang', 'primordial_soup',
     'pack', 'Field', 'lifelike', 'color_life', 'shifting_sands', 'noisetv',
-    'crc32', 'golly2int', 'numpy_life_step', 'WIDE', 'energy_life',
+    'crc32', 'golly2int', 'numpy_life_step', 'WIDE', 'ENERGY_BITS',
+    'energy_life',
     'energy_life_field', 'xorshift32_period', 'energy_fields',
 ]
 
@@ -82,7 +83,8 @@ def library():
             p, p, i64, i64, p, p, i32, i32, i32, i32, i32]
         lib.oracle_noisetv.argtypes = [p, p, i64, i32, i32, i32, i32, i32]
         lib.oracle_energy_life.argtypes = [
-            p, p, i64, i64, p, p, u32, u32, u32, i32, i32, i32, i32, i32, p]
+            p, i32, p, i64, i64, p, p, u32, u32, u32, i32, i32, i32, i32,
+            i32, p]
         lib.oracle_energy_life.restype = i32
         lib.oracle_xorshift32_period.argtypes = [u32]
         lib.oracle_xorshift32_period.restype = ctypes.c_uint64
@@ -257,13 +259,15 @@ def noisetv(field, steps, fade=(255, 255, 1), threads=0):
 
 def energy_life(field, steps, sustain=32, birth=128, mutation=16,
                 fade=(255, 255, 1), colors=True, threads=0, hashes=False):
-    """EnergyLife steps in place on a :data:`WIDE` field."""
+    """EnergyLife steps in place on a field of 64-bit words (the model's
+    layout) or :data:`WIDE` words (its 128-bit variant)."""
     dl = np.array(moore_deltas(2), dtype=np.int8).reshape(-1)
     rv = np.array(moore_rev(2), dtype=np.int32)
     out = np.zeros(max(steps, 1), dtype=np.uint32) if hashes else None
-    assert field.fld.dtype == WIDE
+    wide = field.fld.dtype == WIDE
+    assert wide or field.fld.dtype == np.uint64
     assert library().oracle_energy_life(
-        _ptr(field.fld), _ptr(field.col) if colors else None,
+        _ptr(field.fld), int(wide), _ptr(field.col) if colors else None,
         field.size[0], field.size[1], _ptr(dl), _ptr(rv), sustain, birth,
         mutation, fade[0], fade[1], fade[2], steps, threads, _ptr(out)) == 0
     return out[:steps] if hashes else None
@@ -274,43 +278,59 @@ def xorshift32_period(seed):
     return int(library().oracle_xorshift32_period(seed))
 
 
+# EnergyLife's field layouts (examples/energy_life.py): energy width;
+# genome (18 bits), mode (1) and the xorshift32 state (32) follow it
+ENERGY_BITS = {False: 8, True: 16}
+
+
 def energy_fields(words):
-    """``(energy, genome, mode, rng)`` arrays of :data:`WIDE` words."""
-    lo = words['lo'].astype(np.uint64)
-    hi = words['hi'].astype(np.uint64)
-    energy = (lo & np.uint64(0xFFFF)).astype(np.int64)
-    genome = ((lo >> np.uint64(16)) & np.uint64(0x3FFFF)).astype(np.int64)
-    mode = ((lo >> np.uint64(34)) & np.uint64(1)).astype(np.int64)
-    rng = ((lo >> np.uint64(35)) | (hi << np.uint64(29))) & np.uint64(
-        0xFFFFFFFF)
-    return energy, genome, mode, rng.astype(np.int64)
+    """``(energy, genome, mode, rng)`` arrays of EnergyLife words (uint64,
+    or :data:`WIDE` for the 128-bit variant)."""
+    wide = words.dtype == WIDE
+    eb = ENERGY_BITS[wide]
+    if wide:
+        lo = words['lo'].astype(np.uint64)
+        hi = words['hi'].astype(np.uint64)
+        rng = ((lo >> np.uint64(eb + 19)) | (hi << np.uint64(64 - eb - 19)))
+    else:
+        lo = words.astype(np.uint64)
+        rng = lo >> np.uint64(eb + 19)
+    energy = (lo & np.uint64((1 << eb) - 1)).astype(np.int64)
+    genome = ((lo >> np.uint64(eb)) & np.uint64(0x3FFFF)).astype(np.int64)
+    mode = ((lo >> np.uint64(eb + 18)) & np.uint64(1)).astype(np.int64)
+    rng = (rng & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    return energy, genome, mode, rng
 
 
-def energy_words(energy, genome, mode, rng):
-    """Pack non-negative in-range fields into :data:`WIDE` words."""
+def energy_words(energy, genome, mode, rng, wide=False):
+    """Pack non-negative in-range fields into EnergyLife words."""
     energy, genome, mode, rng = (np.asarray(v, dtype=np.uint64)
                                  for v in (energy, genome, mode, rng))
+    eb = ENERGY_BITS[wide]
+    lo = (energy | (genome << np.uint64(eb)) | (mode << np.uint64(eb + 18))
+          | (rng << np.uint64(eb + 19)))
+    if not wide:
+        return lo
     out = np.zeros(energy.shape, dtype=WIDE)
-    out['lo'] = (energy | (genome << np.uint64(16)) | (mode << np.uint64(34))
-                 | (rng << np.uint64(35)))
-    out['hi'] = rng >> np.uint64(29)
+    out['lo'] = lo
+    out['hi'] = rng >> np.uint64(64 - eb - 19)
     return out
 
 
-def energy_life_field(size, word=42):
+def energy_life_field(size, word=42, wide=False):
     """
-    Step-0 state of ``EnergyLifeExperiment``: ``PrimordialSoup`` drawing,
-    in sorted name order, energy ``RandInt(0, 255)``, genome
-    ``RandInt(0, 2**18 - 1)``, mode ``RandInt(0, 1)`` and generator
-    ``RandInt(1, 2**32 - 1)``.
+    Step-0 state of ``EnergyLifeExperiment`` (``wide``: of the 128-bit
+    variant): ``PrimordialSoup`` drawing, in sorted name order, energy
+    ``RandInt(0, 255)``, genome ``RandInt(0, 2**18 - 1)``, mode
+    ``RandInt(0, 1)`` and generator ``RandInt(1, 2**32 - 1)``.
     """
     rng = local_random(word)
     vals, _ = primordial_soup(size, {
         "energy": randint(0, 255), "genome": randint(0, 2 ** 18 - 1),
         "mode": randint(0, 1), "rng": randint(1, 2 ** 32 - 1)}, rng)
     words = energy_words(vals["energy"], vals["genome"], vals["mode"],
-                         vals["rng"])
-    return Field(size, 8, WIDE, words)
+                         vals["rng"], wide)
+    return Field(size, 8, WIDE if wide else np.uint64, words)
 
 
 def crc32(data):

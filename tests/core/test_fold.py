@@ -5,12 +5,14 @@ Buffer slot ``p`` folds to ``(main & keep) | const`` when emit stores only
 copies of the cell's own main fields and integer constants; anything else
 keeps the buffer planes.
 """
+import pytest
+
 from xentica_amd import core
 from xentica_amd.runtime.fused import FusedProgram
 from examples.base import RegularCA
 from examples.game_of_life import GameOfLife, GOLExperiment
 from examples.shifting_sands import ShiftingSands, ShiftingSandsExperiment
-from examples.energy_life import EnergyLife, EnergyLifeSmall
+from examples.energy_life import EnergyLife, EnergyLifeSmall, EnergyLifeWide
 
 
 class Flipped(RegularCA):
@@ -61,10 +63,13 @@ def test_shifting_sands_folds_to_one_copy_and_zeros():
     assert all(fold[slot] == (0, 0) for slot in range(1, 8))
 
 
-def test_wide_word_fold_keeps_energy_genome_mode():
-    fold = _fold(EnergyLife, EnergyLifeSmall)
-    assert set(fold.values()) == {((1 << 35) - 1, 0)}
-    program = FusedProgram(EnergyLife(EnergyLifeSmall))
+@pytest.mark.parametrize("cls,kept", [(EnergyLife, 27), (EnergyLifeWide, 35)])
+def test_wide_word_fold_keeps_energy_genome_mode(cls, kept):
+    """64- and 128-bit words: buffers are the main word's energy, genome
+    and mode bits (the generator field zeroed)."""
+    fold = _fold(cls, EnergyLifeSmall)
+    assert set(fold.values()) == {((1 << kept) - 1, 0)}
+    program = FusedProgram(cls(EnergyLifeSmall))
     assert program.folded and program.halo_per_step == 1
     assert "_tb" not in program.source
 

@@ -1,6 +1,7 @@
 """
-EnergyLife (the 128-bit EvoLife-style model) on the CPU: the oracle's pins,
-the 128-bit seed packing and the generated code.
+EnergyLife (the EvoLife-style model, 64-bit words, and its 128-bit
+variant) on the CPU: the oracle's pins in both layouts, the seed packing
+and the generated code.
 
 The model has no reference counterpart, so no reference vector exists for
 it; the oracle is pinned by what the mathematics fixes:
@@ -12,24 +13,28 @@ it; the oracle is pinned by what the mathematics fixes:
   mutation, energies >= 1) against the NumPy roll-sum Life;
 * the xorshift32 (13, 17, 5) period 2**32 - 1 (Marsaglia 2003) and one
   advance per step;
-* newborns take a live neighbour's genome (no mutation).
+* newborns take a live neighbour's genome (no mutation);
+* the 64-bit and 128-bit layouts give the same fields step for step.
 """
 import numpy as np
 import pytest
 
 from oracle import oracle
-from examples.energy_life import EnergyLife, EnergyLifeSmall
+from examples.energy_life import EnergyLife, EnergyLifeSmall, EnergyLifeWide
+
+LAYOUTS = pytest.mark.parametrize("wide", [False, True],
+                                  ids=["64bit", "128bit"])
 
 
-def small_field(energy, genome, mode, rng):
+def small_field(energy, genome, mode, rng, wide=False):
     """A 2-D oracle field from ``(w1, w0)`` field arrays."""
     energy = np.asarray(energy)
     size = (energy.shape[1], energy.shape[0])
     words = oracle.energy_words(
         np.ravel(energy), np.ravel(np.broadcast_to(genome, energy.shape)),
         np.ravel(np.broadcast_to(mode, energy.shape)),
-        np.ravel(np.broadcast_to(rng, energy.shape)))
-    return oracle.Field(size, 8, oracle.WIDE, words)
+        np.ravel(np.broadcast_to(rng, energy.shape)), wide)
+    return oracle.Field(size, 8, oracle.WIDE if wide else np.uint64, words)
 
 
 def fields_of(field):
@@ -45,17 +50,18 @@ def numpy_xorshift(x):
     return x
 
 
-def test_seed_matches_model():
-    model = EnergyLife(EnergyLifeSmall)
+@LAYOUTS
+def test_seed_matches_model(wide):
+    model = (EnergyLifeWide if wide else EnergyLife)(EnergyLifeSmall)
     cells = model.gpu.arrays.cells.get()[:model.cells_num]
-    field = oracle.energy_life_field(EnergyLifeSmall.size)
-    assert cells.dtype.itemsize == 16
+    field = oracle.energy_life_field(EnergyLifeSmall.size, wide=wide)
+    assert cells.dtype.itemsize == (16 if wide else 8)
     assert cells.tobytes() == field.cells.tobytes()
 
 
 def test_wide_pack_state_is_modular_sum():
     """128-bit ``pack_state`` = sum of value << shift modulo 2**128."""
-    model = EnergyLife(EnergyLifeSmall)
+    model = EnergyLifeWide(EnergyLifeSmall)
     rng = np.random.default_rng(3)
     state = {
         "energy": rng.integers(-2 ** 40, 2 ** 40, 64),
@@ -72,19 +78,28 @@ def test_wide_pack_state_is_modular_sum():
         assert got == want
 
 
-def test_generated_code_uses_one_128_bit_word():
+def test_generated_code_uses_one_64_bit_word():
     model = EnergyLife(EnergyLifeSmall)
     src = model.cuda_source
-    assert "unsigned __int128 *fld" in src
-    assert model.main.bit_width == 67
+    assert "unsigned long *fld" in src or "unsigned long long *fld" in src
+    assert model.main.bit_width == 59
     assert "_cell_rng = (_cell_rng ^ (_cell_rng << 13));" in src
     assert "_cell_rng = (_cell_rng ^ (_cell_rng >> 17));" in src
     assert "_cell_rng = (_cell_rng ^ (_cell_rng << 5));" in src
+    assert "<< 27)" in src
+
+
+def test_generated_code_of_the_wide_variant_uses_one_128_bit_word():
+    model = EnergyLifeWide(EnergyLifeSmall)
+    src = model.cuda_source
+    assert "unsigned __int128 *fld" in src
+    assert model.main.bit_width == 67
     assert "<< 35)" in src
 
 
-def test_energy_conserved_and_bounded():
-    field = oracle.energy_life_field((96, 80))
+@LAYOUTS
+def test_energy_conserved_and_bounded(wide):
+    field = oracle.energy_life_field((96, 80), wide=wide)
     energy0 = fields_of(field)[0]
     total, low, high = energy0.sum(), energy0.min(), energy0.max()
     for _ in range(10):
@@ -94,16 +109,17 @@ def test_energy_conserved_and_bounded():
         assert low <= energy.min() and energy.max() <= high
 
 
+@LAYOUTS
 @pytest.mark.parametrize("centre_mode,shift", [(0, 3), (1, 4)])
-def test_single_spike(centre_mode, shift):
+def test_single_spike(centre_mode, shift, wide):
     """Spike E on an empty dead field with no births: one step."""
-    big = 1000
+    big = 1000 if wide else 250
     energy = np.zeros((9, 11), dtype=np.int64)
     energy[4, 5] = big
     mode = np.zeros_like(energy)
     mode[4, 5] = centre_mode
     # genome 0: nothing is born; the spike (if live) dies at once
-    field = small_field(energy, 0, mode, 1)
+    field = small_field(energy, 0, mode, 1, wide)
     oracle.energy_life(field, 1, sustain=0, birth=0, mutation=0)
     new, genome, new_mode, _ = fields_of(field)
     share = big >> shift
@@ -114,19 +130,22 @@ def test_single_spike(centre_mode, shift):
     assert not new_mode.any() and not genome.any()
 
 
-def test_uniform_field_is_fixed():
+@LAYOUTS
+def test_uniform_field_is_fixed(wide):
     energy = np.full((8, 8), 77, dtype=np.int64)
-    field = small_field(energy, 0, 0, 5)
+    field = small_field(energy, 0, 0, 5, wide)
     oracle.energy_life(field, 3)
     assert np.array_equal(fields_of(field)[0], energy)
 
 
-def test_reduces_to_life():
+@LAYOUTS
+def test_reduces_to_life(wide):
     """Uniform B3/S23 genome, zero thresholds, no mutation: mode is Life."""
     rng = np.random.default_rng(7)
     energy = rng.integers(1, 256, (48, 64))
     mode = rng.integers(0, 2, (48, 64))
-    field = small_field(energy, oracle.golly2int("B3/S23"), mode, 12345)
+    field = small_field(energy, oracle.golly2int("B3/S23"), mode, 12345,
+                        wide)
     state = mode.astype(np.uint8)
     for _ in range(30):
         oracle.energy_life(field, 1, sustain=0, birth=0, mutation=0)
@@ -134,14 +153,29 @@ def test_reduces_to_life():
         assert np.array_equal(fields_of(field)[2], state)
 
 
-def test_generator_advances_once_per_step():
-    field = oracle.energy_life_field((32, 24))
+@LAYOUTS
+def test_generator_advances_once_per_step(wide):
+    field = oracle.energy_life_field((32, 24), wide=wide)
     rng0 = fields_of(field)[3]
     oracle.energy_life(field, 5)
     want = rng0
     for _ in range(5):
         want = numpy_xorshift(want)
     assert np.array_equal(fields_of(field)[3], want.astype(np.int64))
+
+
+def test_layouts_agree():
+    """The 64-bit and 128-bit layouts hold the same fields after every
+    step of the seeded workload (energies stay within 0-255), colours
+    included."""
+    narrow = oracle.energy_life_field((64, 48))
+    wide = oracle.energy_life_field((64, 48), wide=True)
+    for _ in range(25):
+        oracle.energy_life(narrow, 2)
+        oracle.energy_life(wide, 2)
+        for a, b in zip(fields_of(narrow), fields_of(wide)):
+            assert np.array_equal(a, b)
+        assert np.array_equal(narrow.col, wide.col)
 
 
 def test_xorshift32_period():

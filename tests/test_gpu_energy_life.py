@@ -1,10 +1,11 @@
 """
 EnergyLife on the GPU engines against the oracle.
 
-Per-step CRC-32 of the 128-bit main plane and the colours on a 96x80
-torus (plain and fused engines); the 4096x4096 workload a few steps
-against the oracle; conserved total energy over a longer run, and after
-every one of the workload's 5000 steps (summed on the device).
+Per-step CRC-32 of the main plane (64-bit words, and the 128-bit variant)
+and the colours on a 96x80 torus (plain and fused engines); the 4096x4096
+workload a few steps against the oracle; conserved total energy over a
+longer run, and after every one of the workload's 5000 steps (summed on
+the device).
 """
 import binascii
 
@@ -13,8 +14,8 @@ import pytest
 
 from oracle import oracle
 from examples.energy_life import (
-    EnergyLife, EnergyLifeExperiment, EnergyLifeSmall, field_position,
-    field_values)
+    EnergyLife, EnergyLifeExperiment, EnergyLifeSmall, EnergyLifeWide,
+    field_position, field_values)
 from xentica_amd.runtime import kernels
 
 pytestmark = pytest.mark.gpu
@@ -28,11 +29,13 @@ def model_cells(model):
     return model.gpu.arrays.cells.get()[:model.cells_num]
 
 
+@pytest.mark.parametrize("wide", [False, True], ids=["64bit", "128bit"])
 @pytest.mark.parametrize("engine", ["plain", "fused", "fused_unfolded",
                                     "fused_generic"])
-def test_small_per_step(engine):
-    model = EnergyLife(experiment(EnergyLifeSmall, engine))
-    field = oracle.energy_life_field(EnergyLifeSmall.size)
+def test_small_per_step(engine, wide):
+    model = (EnergyLifeWide if wide else EnergyLife)(
+        experiment(EnergyLifeSmall, engine))
+    field = oracle.energy_life_field(EnergyLifeSmall.size, wide=wide)
     hashes = oracle.energy_life(field, 40, hashes=True)
     for step in range(40):
         model.step()
@@ -113,20 +116,22 @@ def test_energy_conserved_every_step_of_the_workload():
         assert device_energy(model) == total, "after step %d" % (step + 1)
 
 
-def test_batch_of_uneven_fields():
-    """128-bit words in one persistent launch: every instance = its oracle."""
+@pytest.mark.parametrize("wide", [False, True], ids=["64bit", "128bit"])
+def test_batch_of_uneven_fields(wide):
+    """64-bit (and 128-bit) words in one persistent launch: every instance
+    equals its oracle."""
     from xentica_amd.runtime.fused import Batch
     sizes = [(96, 80), (64, 200), (130, 48)]
     exps = [type("Energy%d" % idx, (EnergyLifeSmall, ),
                  {"size": size, "word": 42 + idx})
             for idx, size in enumerate(sizes)]
-    models = [EnergyLife(exp) for exp in exps]
+    models = [(EnergyLifeWide if wide else EnergyLife)(exp) for exp in exps]
     batch = Batch(models, tile=(16, 16), depth=2)
     batch.run(11)
     batch.synchronize()
     batch.scatter()
     for model, exp in zip(models, exps):
-        field = oracle.energy_life_field(exp.size, exp.word)
+        field = oracle.energy_life_field(exp.size, exp.word, wide=wide)
         oracle.energy_life(field, 11)
         assert model_cells(model).tobytes() == field.cells.tobytes(), exp.size
         assert np.array_equal(model.gpu.arrays.colors.get(), field.col)
