@@ -962,11 +962,13 @@ def main():
                    "(bits)",
             "size": list(SIZE), "rule": "B3/S23", "seed_word": WORD,
             "density": 0.5, "temporal_depth": engine.depth,
-            "row_strips": engine.strips or "library default (368 at "
-                                           "16384^2: eight 62-word tiles "
-                                           "and a third of a shared "
-                                           "remainder wave per strip, three "
-                                           "waves per SIMD)",
+            "row_strips": engine.strips or "library default (364 at "
+                                           "16384^2: 91 workgroups of four "
+                                           "strips handing rows over in "
+                                           "LDS, eight 62-word tiles per "
+                                           "strip, the 16-word remainder "
+                                           "in segments shared three per "
+                                           "wave, three waves per SIMD)",
             "layout": "64 cells per uint64 (32 per uint32 lane word)",
             "device": device.name,
         },
