@@ -4,9 +4,10 @@ CPU emulation of ``life_bits_kernel`` (``xentica_amd/csrc/life_bits.hip``).
 Test infrastructure: it mirrors the kernel's control flow line by line
 (the wave plan: whole 62-column tiles over balanced row strips and the
 remainder column's segments, interior ones sharing a wave; the K-level
-row pipeline with its prologue, steady state of nine rows requested a
-whole iteration ahead and remainder, the row wrapping, DPP wave shifts
-with zero fill at the wave edges, and stores dropped by the buffer range
+row pipeline -- rows in threes for any rule, in pairs for B3/S23 -- with
+its prologue, steady state of nine (pairs: ten) rows requested a whole
+iteration ahead and remainder, the row wrapping, DPP wave shifts with
+zero fill at the wave edges, and stores dropped by the buffer range
 check) with the 64 lanes of a wavefront as
 NumPy vectors, so the kernel's termination, bounds and results can be
 checked on a machine without a GPU.
@@ -59,8 +60,26 @@ def life_word(a, b, c, born, sustain, fast):
         res |= eq & ((alive & live_next) | ((~alive & M32) & dead_next))
     return res
 
+def pair_sum(r0, r1, s0, s1):
+    """``pair_sum``: p0 + 2 p1 + 4 p2 = (r0 + 2 r1) + (s0 + 2 s1)."""
+    c = r0 & s0
+    return r0 ^ s0, r1 ^ s1 ^ c, maj(r1, s1, c)
+
+
+def pair_life(a0, a1, p0, p1, p2, m):
+    """``pair_life``: B3/S23 from a row's horizontal sum (a0, a1), the
+    pair sum (p0, p1, p2) and the cell's state m, in the kernel's four
+    three-input operations."""
+    o1 = (a0 ^ p0 ^ m) & (~maj(a0, p0, m) & M32)           # one of three
+    o2 = (a1 ^ p1) | (a1 & p1 & p2)
+    z = (~(m ^ o1 ^ o2) & M32) & maj(m, o1, o2)           # two of three
+    return z & (~(p2 & o2) & M32)
+
+
 OUT_LANES = 62
-AHEAD = 9
+AHEAD = 10        # kAhead: the most rows a wave requests past its window
+ROWS_TRIPLE = 9   # rows fed per steady-state iteration, rows in threes
+ROWS_PAIR = 10    # the same, B3/S23 rows in pairs
 MAX_GROUP = 4
 G = 4           # waves per workgroup
 MIN_EXCHANGE_DEPTH = 4
@@ -170,6 +189,13 @@ class Wave:
         self.L = out + 2 if self.exchange else out + 2 * K
         self.row0 = row_lo - K
         self.lev = [[None] * 3 for _ in range(K + 1)]
+        self.rows = ROWS_PAIR if self.fast else ROWS_TRIPLE
+        zero = np.zeros(64, dtype=np.uint64)
+        # PairPipe: q0..q2, s0, s1, c of each level 1 .. K
+        self.pq = [[zero, zero, zero] for _ in range(K + 1)]
+        self.ps = [[zero, zero] for _ in range(K + 1)]
+        self.pc = [zero for _ in range(K + 1)]
+        self.last = None
         self.exports = {}
 
     def load_row(self, i):
@@ -207,8 +233,53 @@ class Wave:
                       self.sustain, self.fast)
         self.lev[l][j3] = hrow(v) if l < K else (v, None, None)
 
+    def take(self, l, x, odd, pair, out):
+        """``PairPipe::take``: level l takes a row of level l - 1 (odd: an
+        odd row n) and returns its own row n - 1 when ``out``."""
+        v = None
+        if odd:
+            zero = np.zeros(64, dtype=np.uint64)
+            p = pair_sum(self.pq[l][0], self.pq[l][1], x[1], x[2]) if pair \
+                else (zero, zero, zero)
+            if out:
+                v = pair_life(self.ps[l][0], self.ps[l][1], *p, self.pc[l])
+            self.pq[l] = list(p)
+            self.ps[l] = [x[1], x[2]]
+        else:
+            if out:
+                v = pair_life(x[1], x[2], *self.pq[l], self.pc[l])
+            self.pq[l] = [x[1], x[2], self.pq[l][2]]
+        self.pc[l] = x[0]
+        return v
+
+    def feed_pairs(self, i, inp, all_levels):
+        """``PairPipe::feed``: level l takes row i - l + 1 of level l - 1
+        from i = 2l - 2 on and completes its row i - l from i = 2l on."""
+        K = self.K
+        x = hrow(inp)
+        for l in range(1, K + 1):
+            if not all_levels and i < 2 * l - 2:
+                break
+            v = self.take(l, x, (i - l + 1) % 2 == 1,
+                          all_levels or i >= 2 * l - 1, all_levels or i >= 2 * l)
+            if not all_levels and i < 2 * l:
+                break
+            if l < K:
+                x = hrow(v)
+            else:
+                self.last = v
+
+    def output(self, i):
+        """The level-K row completed by the feed of row i."""
+        if self.fast:
+            return self.last
+        return self.lev[self.K][(i - self.K) % 3][0]
+
     def feed(self, i, inp, all_levels):
         assert i < self.L
+        if self.fast:
+            self.feed_pairs(i, inp, all_levels)
+            return
         self.lev[0][i % 3] = hrow(inp)
         for l in range(1, self.K + 1):
             if all_levels or i >= 2 * l:
@@ -218,31 +289,34 @@ class Wave:
         K = self.K
         self.pro = [self.load_row(r) for r in range(2 * K)]
         self.i = 2 * K
-        self.cur = [self.load_row(self.i + r) for r in range(AHEAD)]
+        self.cur = [self.load_row(self.i + r) for r in range(self.rows)]
         for r in range(2 * K):
             self.feed(r, self.pro[r], False)
             if r >= 2:
                 m = r // 2
-                self.exports[(m, r & 1)] = self.lev[m][(r - m) % 3][0].copy()
+                row = self.pc[m + 1] if self.fast else \
+                    self.lev[m][(r - m) % 3][0]
+                self.exports[(m, r & 1)] = row.copy()
 
     def finish(self, successor):
         K, L = self.K, self.L
         i, cur = self.i, self.cur
         iters = 0
-        while i + AHEAD - 1 < L:
-            nxt = [self.load_row(i + AHEAD + r) for r in range(AHEAD)]
-            for q in range(AHEAD):
+        rows = self.rows
+        while i + rows - 1 < L:
+            nxt = [self.load_row(i + rows + r) for r in range(rows)]
+            for q in range(rows):
                 self.feed(i + q, cur[q], True)
-                self.store_row(self.lev[K][(i + q - K) % 3][0], i + q - 2 * K)
+                self.store_row(self.output(i + q), i + q - 2 * K)
             cur = nxt
-            i += AHEAD
+            i += rows
             iters += 1
             assert iters < 100000
-        for r in range(AHEAD):
+        for r in range(rows):
             if i + r >= L:
                 break
             self.feed(i + r, cur[r], True)
-            self.store_row(self.lev[K][(i + r - K) % 3][0], i + r - 2 * K)
+            self.store_row(self.output(i + r), i + r - 2 * K)
         if not self.exchange or K == 1:
             return
         # drain: feed D = 2 .. 2K-1 takes level D // 2's row from the next
@@ -252,10 +326,19 @@ class Wave:
         for D in range(2, 2 * K):
             i = out + D
             mid = D // 2
-            self.lev[mid][(i - mid) % 3] = hrow(successor.exports[(mid, D % 2)])
-            for l in range(mid + 1, K + 1):
-                self.compute(l, i)
-            self.store_row(self.lev[K][(i - K) % 3][0], i - 2 * K)
+            x = hrow(successor.exports[(mid, D % 2)])
+            if self.fast:
+                for l in range(mid + 1, K + 1):
+                    v = self.take(l, x, (i - l + 1) % 2 == 1, True, True)
+                    if l < K:
+                        x = hrow(v)
+                    else:
+                        self.last = v
+            else:
+                self.lev[mid][(i - mid) % 3] = x
+                for l in range(mid + 1, K + 1):
+                    self.compute(l, i)
+            self.store_row(self.output(i), i - 2 * K)
 
 
 def kernel(src, W, H, plan, K, born, sustain, writes):

@@ -44,10 +44,6 @@ constexpr int kWavesPerGroup = 4;
 // made those waves the launch's slowest (2048^2: 0.90 and 1.07 us per step
 // against 0.83 at 4 and 0.79 with none shared; 16384^2 gains 6 % at 3)
 constexpr int kMaxGroup = 4;
-// rows requested ahead of the row being fed: the steady state loads the
-// nine rows of the next iteration, so a wave reads up to row L + 8 of its
-// window of L rows
-constexpr int kAhead = 9;
 // Depth from which whole tiles run as exchanging groups: the halo rows
 // they save grow as K (K - 1), and at depths 1 and 2 the plain strips
 // measured faster (16384^2: 15.2 and 15.9 us per launch against 17.6 and
@@ -130,6 +126,104 @@ __device__ __forceinline__ void drain_all(Window<K> &w, const uint32_t *imported
             constexpr int I3 = (S + D) % 3;
             drain_feed<K, kFast, I3, D>(w, imported[Ds], rule);
             store(level_k_row<K, I3>(w), out + D - 2 * K);
+        }(),
+        ...);
+}
+
+// Row pipeline of B3/S23 with rows taken in pairs (pair_life).  Rows of
+// every level are numbered from the tile's first input row, and level l
+// takes row n of level l - 1 with feed n + l - 1, so the parity of n is
+// fixed in every unrolled feed.  Of the rows it has taken, level l keeps
+//   after an odd row n:  the pair sum of rows n - 1 and n (q0, q1, q2) and
+//                        the horizontal sum of row n (s0, s1);
+//   after an even row n: the horizontal sums of rows n (q0, q1) and n - 1
+//                        (s0, s1);
+// and the state of row n (c).  Row n then completes level l's row n - 1:
+// from the single row n - 2 and the new pair for odd n, from the pair kept
+// and the single row n for even n.  Six registers a level, against nine
+// for three rows with their sums.
+template <int K>
+struct PairPipe {
+    static constexpr int kRows = 10;  // rows fed per steady-state iteration
+    uint32_t q0[K + 1], q1[K + 1], q2[K + 1], s0[K + 1], s1[K + 1], c[K + 1];
+    uint32_t last = 0;  // level K's row of the latest feed
+
+    // Level l takes row n of level l - 1 and returns its row n - 1 when
+    // `out` (n - 1 >= l); `pair`: row n - 1 of level l - 1 is valid.
+    __device__ __forceinline__ uint32_t take(int l, const HRow &x, bool odd, bool pair,
+                                             bool out) {
+        uint32_t v = 0;
+        if (odd) {
+            uint32_t p0 = 0, p1 = 0, p2 = 0;
+            if (pair) pair_sum(q0[l], q1[l], x.h0, x.h1, p0, p1, p2);
+            if (out) v = pair_life(s0[l], s1[l], p0, p1, p2, c[l]);
+            q0[l] = p0;
+            q1[l] = p1;
+            q2[l] = p2;
+            s0[l] = x.h0;
+            s1[l] = x.h1;
+        } else {
+            if (out) v = pair_life(x.h0, x.h1, q0[l], q1[l], q2[l], c[l]);
+            q0[l] = x.h0;
+            q1[l] = x.h1;
+        }
+        c[l] = x.raw;
+        return v;
+    }
+    // Feed input row i (i % 2 == S): level l takes row i - l + 1 of level
+    // l - 1 once that row exists (i >= 2l - 2) and completes its own row
+    // i - l from i = 2l on.
+    template <int S, bool kAll>
+    __device__ __forceinline__ void feed(uint32_t input, int i) {
+        HRow x = hrow(input);
+#pragma unroll
+        for (int l = 1; l <= K; ++l) {
+            if (!kAll && i < 2 * l - 2) break;
+            const uint32_t v =
+                take(l, x, ((S + l + 1) & 1) != 0, kAll || i >= 2 * l - 1, kAll || i >= 2 * l);
+            if (!kAll && i < 2 * l) break;
+            if (l < K)
+                x = hrow(v);
+            else
+                last = v;
+        }
+    }
+    // level m's row completed by feed r (m < K): level m + 1 took it
+    __device__ __forceinline__ uint32_t level_row(int m) const { return c[m + 1]; }
+    // Drain feed D of a tile that hands its last rows over (as drain_feed;
+    // S = (out + D) % 2, the parity of the feed's row).
+    template <int S, int D>
+    __device__ __forceinline__ void drain(uint32_t imported) {
+        constexpr int kMid = D / 2;
+        HRow x = hrow(imported);
+#pragma unroll
+        for (int l = kMid + 1; l <= K; ++l) {
+            const uint32_t v = take(l, x, ((S + l + 1) & 1) != 0, true, true);
+            if (l < K)
+                x = hrow(v);
+            else
+                last = v;
+        }
+    }
+};
+
+// Rows requested ahead of the row being fed: the steady state loads the
+// rows of its next iteration (nine for three-row slots, ten for pairs), so
+// a wave reads up to row L + kAhead - 1 of its window of L rows.
+constexpr int kAhead = 10;
+static_assert(PairPipe<1>::kRows <= kAhead, "rows requested ahead");
+
+// The 2K - 2 drain feeds of a B3/S23 tile whose out % 2 == S, each
+// storing the level-K row it completes (output row out + D - 2K).
+template <int K, int S, typename Store, int... Ds>
+__device__ __forceinline__ void drain_pairs(PairPipe<K> &pipe, const uint32_t *imported,
+                                            Store &store, int out,
+                                            std::integer_sequence<int, Ds...>) {
+    (
+        [&] {
+            constexpr int D = Ds + 2;
+            pipe.template drain<(S + D) % 2, D>(imported[Ds]);
+            store(pipe.last, out + D - 2 * K);
         }(),
         ...);
 }
@@ -320,107 +414,177 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
                                               keep ? (out_row0 + r) * row_bytes : 0, 0);
     };
 
-    Rule<kFast> rule{born, sustain};
-    Window<K> w;
-    // The steady state feeds nine rows per iteration from the registers
-    // `cur`; at the top of each iteration the nine rows of the next one are
-    // requested into `nxt`, and `cur = nxt` moves them over at the end.  The
-    // compiler turns that hand-over (and, in practice, any other order of
-    // loads across the back edge) into register copies at the loop head,
-    // and a copy waits for its load: here those loads were issued a whole
-    // iteration (72 row-levels of work) earlier, so the wait is normally
-    // already satisfied and the stores stay in flight.  The prologue's 2K
-    // rows and the first nine are all requested before the first feed.
-    // Rows past L-1 are wrapped rows: in bounds, loaded and never fed.
-    constexpr int P0 = (2 * K) % 3, P1 = (2 * K + 1) % 3, P2 = (2 * K + 2) % 3;
-    int i = 2 * K;
-    uint32_t cur[9], nxt[9];
-    uint32_t *const mine = &xch[wig][0][0][lane];
-    if (live) {
-        uint32_t pro[2 * K];
+    if constexpr (kFast) {
+        // B3/S23: rows in pairs, ten rows per steady-state iteration (an
+        // even count keeps each feed's row parity fixed); otherwise as for
+        // any rule below
+        constexpr int kRows = PairPipe<K>::kRows;
+        PairPipe<K> pipe;
+        int i = 2 * K;
+        uint32_t cur[kRows], nxt[kRows];
+        uint32_t *const mine = &xch[wig][0][0][lane];
+        if (live) {
+            uint32_t pro[2 * K];
 #pragma unroll
-        for (int r = 0; r < 2 * K; ++r) pro[r] = load_next();
+            for (int r = 0; r < 2 * K; ++r) pro[r] = load_next();
 #pragma unroll
-        for (int r = 0; r < 9; ++r) cur[r] = load_next();
-        // prologue: rows 0 .. 2K-1 fill the pipeline (no output yet); feed r
-        // completes row r - l of each level l <= r / 2, the first two rows of
-        // level r / 2 at r = 2l and 2l + 1: those go to LDS
+            for (int r = 0; r < kRows; ++r) cur[r] = load_next();
 #pragma unroll
-        for (int r = 0; r < 2 * K; ++r) {
-            switch (r % 3) {
-                case 0: feed<K, kFast, 0, false>(w, pro[r], r, rule); break;
-                case 1: feed<K, kFast, 1, false>(w, pro[r], r, rule); break;
-                default: feed<K, kFast, 2, false>(w, pro[r], r, rule); break;
-            }
-            if (r >= 2) {
-                const int m = r / 2;
-                mine[((m - 1) * 2 + (r & 1)) * kWave] = w.lev[m][((r - m) % 3 + 3) % 3].raw;
+            for (int r = 0; r < 2 * K; ++r) {
+                if (r % 2 == 0)
+                    pipe.template feed<0, false>(pro[r], r);
+                else
+                    pipe.template feed<1, false>(pro[r], r);
+                if (r >= 2) {
+                    const int m = r / 2;
+                    mine[((m - 1) * 2 + (r & 1)) * kWave] = pipe.level_row(m);
+                }
             }
         }
-    }
-    __syncthreads();
-    if (!live) return;
-    // rows row .. row+2 (slots P0, P1, P2 by row mod 3), each output row
-    // r = row - 2K stored as its level-K row leaves the pipeline
-    auto feed3 = [&](uint32_t x0, uint32_t x1, uint32_t x2, int row) {
-        feed<K, kFast, P0, true>(w, x0, row, rule);
-        store_row(level_k_row<K, P0>(w), row - 2 * K);
-        feed<K, kFast, P1, true>(w, x1, row + 1, rule);
-        store_row(level_k_row<K, P1>(w), row + 1 - 2 * K);
-        feed<K, kFast, P2, true>(w, x2, row + 2, rule);
-        store_row(level_k_row<K, P2>(w), row + 2 - 2 * K);
-    };
-    // Every load so far has been consumed or is long complete: wait for them
-    // here, so that the loop head's waits are counted from the previous
-    // iteration alone (the compiler merges the entry path into the loop
-    // head's count, and the prologue's load order would otherwise make each
-    // hand-over wait for the previous iteration's stores as well).
-    __builtin_amdgcn_s_waitcnt(kVmcntZero);
-    for (; i + 8 < L; i += 9) {
+        __syncthreads();
+        if (!live) return;
+        // rows row (even) and row + 1
+        auto feed2 = [&](uint32_t x0, uint32_t x1, int row) {
+            pipe.template feed<0, true>(x0, row);
+            store_row(pipe.last, row - 2 * K);
+            pipe.template feed<1, true>(x1, row + 1);
+            store_row(pipe.last, row + 1 - 2 * K);
+        };
+        __builtin_amdgcn_s_waitcnt(kVmcntZero);
+        for (; i + kRows - 1 < L; i += kRows) {
 #pragma unroll
-        for (int r = 0; r < 9; ++r) nxt[r] = load_next();  // rows i + 9 .. i + 17
-        // keep the nine loads at the top of the iteration: the scheduler
-        // otherwise sinks them towards their use, and the hand-over at the
-        // loop head then waits on loads issued late in the iteration
-        __builtin_amdgcn_sched_barrier(0);
-        feed3(cur[0], cur[1], cur[2], i);
-        feed3(cur[3], cur[4], cur[5], i + 3);
-        feed3(cur[6], cur[7], cur[8], i + 6);
+            for (int r = 0; r < kRows; ++r) nxt[r] = load_next();
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int r = 0; r < 9; ++r) cur[r] = nxt[r];
-    }
-    // the last 0 .. 8 rows
+            for (int r = 0; r < kRows; r += 2) feed2(cur[r], cur[r + 1], i + r);
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        if (i + r >= L) break;
-        switch ((2 * K + r) % 3) {
-            case 0:
-                feed<K, kFast, 0, true>(w, cur[r], i + r, rule);
-                store_row(level_k_row<K, 0>(w), i + r - 2 * K);
-                break;
-            case 1:
-                feed<K, kFast, 1, true>(w, cur[r], i + r, rule);
-                store_row(level_k_row<K, 1>(w), i + r - 2 * K);
-                break;
-            default:
-                feed<K, kFast, 2, true>(w, cur[r], i + r, rule);
-                store_row(level_k_row<K, 2>(w), i + r - 2 * K);
-                break;
+            for (int r = 0; r < kRows; ++r) cur[r] = nxt[r];
         }
-    }
-    if constexpr (K > 1) {
-        if (!exchange) return;
-        // drain: the next strip's first two rows of each level 1 .. K-1, in
-        // the order the feeds D = 2 .. 2K-1 take them (level D / 2, row D % 2)
-        uint32_t imported[2 * K - 2];
-        const uint32_t *const next = &xch[wig + 1][0][0][lane];
+        // the last 0 .. kRows - 1 rows
 #pragma unroll
-        for (int d = 0; d < 2 * K - 2; ++d) imported[d] = next[d * kWave];
-        using Feeds = std::make_integer_sequence<int, 2 * K - 2>;
-        switch (out % 3) {
-            case 0: drain_all<K, kFast, 0>(w, imported, rule, store_row, out, Feeds{}); break;
-            case 1: drain_all<K, kFast, 1>(w, imported, rule, store_row, out, Feeds{}); break;
-            default: drain_all<K, kFast, 2>(w, imported, rule, store_row, out, Feeds{}); break;
+        for (int r = 0; r < kRows; ++r) {
+            if (i + r >= L) break;
+            if (r % 2 == 0)
+                pipe.template feed<0, true>(cur[r], i + r);
+            else
+                pipe.template feed<1, true>(cur[r], i + r);
+            store_row(pipe.last, i + r - 2 * K);
+        }
+        if constexpr (K > 1) {
+            if (!exchange) return;
+            uint32_t imported[2 * K - 2];
+            const uint32_t *const next = &xch[wig + 1][0][0][lane];
+#pragma unroll
+            for (int d = 0; d < 2 * K - 2; ++d) imported[d] = next[d * kWave];
+            using Feeds = std::make_integer_sequence<int, 2 * K - 2>;
+            if (out % 2 == 0)
+                drain_pairs<K, 0>(pipe, imported, store_row, out, Feeds{});
+            else
+                drain_pairs<K, 1>(pipe, imported, store_row, out, Feeds{});
+        }
+    } else {
+        Rule<kFast> rule{born, sustain};
+        Window<K> w;
+        // The steady state feeds nine rows per iteration from the registers
+        // `cur`; at the top of each iteration the nine rows of the next one are
+        // requested into `nxt`, and `cur = nxt` moves them over at the end.  The
+        // compiler turns that hand-over (and, in practice, any other order of
+        // loads across the back edge) into register copies at the loop head,
+        // and a copy waits for its load: here those loads were issued a whole
+        // iteration (72 row-levels of work) earlier, so the wait is normally
+        // already satisfied and the stores stay in flight.  The prologue's 2K
+        // rows and the first nine are all requested before the first feed.
+        // Rows past L-1 are wrapped rows: in bounds, loaded and never fed.
+        constexpr int P0 = (2 * K) % 3, P1 = (2 * K + 1) % 3, P2 = (2 * K + 2) % 3;
+        int i = 2 * K;
+        uint32_t cur[9], nxt[9];
+        uint32_t *const mine = &xch[wig][0][0][lane];
+        if (live) {
+            uint32_t pro[2 * K];
+#pragma unroll
+            for (int r = 0; r < 2 * K; ++r) pro[r] = load_next();
+#pragma unroll
+            for (int r = 0; r < 9; ++r) cur[r] = load_next();
+            // prologue: rows 0 .. 2K-1 fill the pipeline (no output yet); feed r
+            // completes row r - l of each level l <= r / 2, the first two rows of
+            // level r / 2 at r = 2l and 2l + 1: those go to LDS
+#pragma unroll
+            for (int r = 0; r < 2 * K; ++r) {
+                switch (r % 3) {
+                    case 0: feed<K, kFast, 0, false>(w, pro[r], r, rule); break;
+                    case 1: feed<K, kFast, 1, false>(w, pro[r], r, rule); break;
+                    default: feed<K, kFast, 2, false>(w, pro[r], r, rule); break;
+                }
+                if (r >= 2) {
+                    const int m = r / 2;
+                    mine[((m - 1) * 2 + (r & 1)) * kWave] = w.lev[m][((r - m) % 3 + 3) % 3].raw;
+                }
+            }
+        }
+        __syncthreads();
+        if (!live) return;
+        // rows row .. row+2 (slots P0, P1, P2 by row mod 3), each output row
+        // r = row - 2K stored as its level-K row leaves the pipeline
+        auto feed3 = [&](uint32_t x0, uint32_t x1, uint32_t x2, int row) {
+            feed<K, kFast, P0, true>(w, x0, row, rule);
+            store_row(level_k_row<K, P0>(w), row - 2 * K);
+            feed<K, kFast, P1, true>(w, x1, row + 1, rule);
+            store_row(level_k_row<K, P1>(w), row + 1 - 2 * K);
+            feed<K, kFast, P2, true>(w, x2, row + 2, rule);
+            store_row(level_k_row<K, P2>(w), row + 2 - 2 * K);
+        };
+        // Every load so far has been consumed or is long complete: wait for them
+        // here, so that the loop head's waits are counted from the previous
+        // iteration alone (the compiler merges the entry path into the loop
+        // head's count, and the prologue's load order would otherwise make each
+        // hand-over wait for the previous iteration's stores as well).
+        __builtin_amdgcn_s_waitcnt(kVmcntZero);
+        for (; i + 8 < L; i += 9) {
+#pragma unroll
+            for (int r = 0; r < 9; ++r) nxt[r] = load_next();  // rows i + 9 .. i + 17
+            // keep the nine loads at the top of the iteration: the scheduler
+            // otherwise sinks them towards their use, and the hand-over at the
+            // loop head then waits on loads issued late in the iteration
+            __builtin_amdgcn_sched_barrier(0);
+            feed3(cur[0], cur[1], cur[2], i);
+            feed3(cur[3], cur[4], cur[5], i + 3);
+            feed3(cur[6], cur[7], cur[8], i + 6);
+#pragma unroll
+            for (int r = 0; r < 9; ++r) cur[r] = nxt[r];
+        }
+        // the last 0 .. 8 rows
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            if (i + r >= L) break;
+            switch ((2 * K + r) % 3) {
+                case 0:
+                    feed<K, kFast, 0, true>(w, cur[r], i + r, rule);
+                    store_row(level_k_row<K, 0>(w), i + r - 2 * K);
+                    break;
+                case 1:
+                    feed<K, kFast, 1, true>(w, cur[r], i + r, rule);
+                    store_row(level_k_row<K, 1>(w), i + r - 2 * K);
+                    break;
+                default:
+                    feed<K, kFast, 2, true>(w, cur[r], i + r, rule);
+                    store_row(level_k_row<K, 2>(w), i + r - 2 * K);
+                    break;
+            }
+        }
+        if constexpr (K > 1) {
+            if (!exchange) return;
+            // drain: the next strip's first two rows of each level 1 .. K-1, in
+            // the order the feeds D = 2 .. 2K-1 take them (level D / 2, row D % 2)
+            uint32_t imported[2 * K - 2];
+            const uint32_t *const next = &xch[wig + 1][0][0][lane];
+#pragma unroll
+            for (int d = 0; d < 2 * K - 2; ++d) imported[d] = next[d * kWave];
+            using Feeds = std::make_integer_sequence<int, 2 * K - 2>;
+            switch (out % 3) {
+                case 0: drain_all<K, kFast, 0>(w, imported, rule, store_row, out, Feeds{}); break;
+                case 1: drain_all<K, kFast, 1>(w, imported, rule, store_row, out, Feeds{}); break;
+                default: drain_all<K, kFast, 2>(w, imported, rule, store_row, out, Feeds{}); break;
+            }
         }
     }
 }

@@ -122,4 +122,63 @@ __device__ __forceinline__ uint32_t life_word(const HRow &a, const HRow &b, cons
     return rule.apply(t0, u, k, v, b.raw);
 }
 
+// B3/S23 over rows taken in pairs.  Output rows 2q and 2q + 1 share the
+// sum of the horizontal sums of rows 2q and 2q + 1,
+//   P = p0 + 2 p1 + 4 p2  (0 .. 6),
+// and add one more row's horizontal sum a0 + 2 a1: row 2q - 1 for output
+// 2q, row 2q + 2 for output 2q + 1.  With count9 = a0 + 2 a1 + P (the cell
+// included) the new state is (count9 == 3) | (m & count9 == 4), m the
+// cell's state.  The four operations below compute it; they were found by
+// an exhaustive search over circuits of three-input logic ops, and
+// `pair_rule_holds` checks them against the rule for every input that can
+// occur: P <= 6, and m = 1 implies P >= 1 (the pair holds the cell itself).
+// A pair costs 4 ops and each output 4, so a row costs 6 + 2 + 4 = 12
+// VALU ops with its horizontal sums against 6 + 8 = 14 for three rows.
+constexpr uint8_t kOneOf3 = bitop3_table([](bool a, bool b, bool c) { return a + b + c == 1; });
+constexpr uint8_t kTwoOf3 = bitop3_table([](bool a, bool b, bool c) { return a + b + c == 2; });
+constexpr uint8_t kPairTwos =
+    bitop3_table([](bool a1, bool p1, bool p2) { return (a1 != p1) || (a1 && p1 && p2); });
+constexpr uint8_t kZNotBoth =
+    bitop3_table([](bool p2, bool o2, bool z) { return z && !(p2 && o2); });
+
+constexpr bool table_bit(uint8_t t, bool a, bool b, bool c) {
+    return (t >> ((a << 2) | (b << 1) | c)) & 1;
+}
+
+constexpr bool pair_rule_holds() {
+    for (int a = 0; a < 4; ++a)
+        for (int P = 0; P <= 6; ++P)
+            for (int m = 0; m < 2; ++m) {
+                if (m && P == 0) continue;
+                const bool a0 = a & 1, a1 = a >> 1, p0 = P & 1, p1 = (P >> 1) & 1, p2 = P >> 2;
+                const bool o1 = table_bit(kOneOf3, a0, p0, m);
+                const bool o2 = table_bit(kPairTwos, a1, p1, p2);
+                const bool z = table_bit(kTwoOf3, m, o1, o2);
+                const int c9 = a + P;
+                if (table_bit(kZNotBoth, p2, o2, z) != (c9 == 3 || (m && c9 == 4))) return false;
+            }
+    return true;
+}
+static_assert(pair_rule_holds(), "paired B3/S23 circuit");
+
+// p0 + 2 p1 + 4 p2 = (r0 + 2 r1) + (s0 + 2 s1): the pair sum of two rows'
+// horizontal sums.
+__device__ __forceinline__ void pair_sum(uint32_t r0, uint32_t r1, uint32_t s0, uint32_t s1,
+                                         uint32_t &p0, uint32_t &p1, uint32_t &p2) {
+    const uint32_t c = r0 & s0;
+    p0 = r0 ^ s0;
+    p1 = bitop3<kXor3>(r1, s1, c);
+    p2 = bitop3<kMaj3>(r1, s1, c);
+}
+
+// New B3/S23 state from a row's horizontal sum (a0, a1), the pair sum
+// (p0, p1, p2) and the cell's state m.
+__device__ __forceinline__ uint32_t pair_life(uint32_t a0, uint32_t a1, uint32_t p0, uint32_t p1,
+                                              uint32_t p2, uint32_t m) {
+    const uint32_t o1 = bitop3<kOneOf3>(a0, p0, m);
+    const uint32_t o2 = bitop3<kPairTwos>(a1, p1, p2);
+    const uint32_t z = bitop3<kTwoOf3>(m, o1, o2);
+    return bitop3<kZNotBoth>(p2, o2, z);
+}
+
 }  // namespace
