@@ -59,9 +59,13 @@ struct LaneRule {
         const uint32_t w0 = u ^ k;
         const uint32_t w1 = bitop3<kXorAnd>(v, u, k);
         const uint32_t w2 = bitop3<kAnd3>(v, u, k);
+        // count9 counts the cell: 0 only for a dead cell, 9 only for a live
+        // one, so those two leaves need no select
         uint32_t leaf[10];
+        leaf[0] = dead_next[0];
+        leaf[9] = alive_next[9];
 #pragma unroll
-        for (int c = 0; c < 10; ++c) leaf[c] = bitop3<kMux>(m, alive_next[c], dead_next[c]);
+        for (int c = 1; c < 9; ++c) leaf[c] = bitop3<kMux>(m, alive_next[c], dead_next[c]);
         const uint32_t l0 = bitop3<kMux>(t0, leaf[1], leaf[0]);
         const uint32_t l1 = bitop3<kMux>(t0, leaf[3], leaf[2]);
         const uint32_t l2 = bitop3<kMux>(t0, leaf[5], leaf[4]);

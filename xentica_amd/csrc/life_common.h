@@ -89,14 +89,16 @@ struct Rule<false> {
         const uint32_t w2 = bitop3<kAnd3>(v, u, k);
         // leaf[c]: next state for count9 = c; a dead cell has c neighbours,
         // a live one c - 1 and is kept when that is a birth OR a survival
-        // count (next = born | sustained & state)
+        // count (next = born | sustained & state).  count9 counts the cell,
+        // so count9 = 0 only for a dead cell and count9 = 9 only for a live
+        // one: those two leaves need no select.
         uint32_t leaf[10];
 #pragma unroll
         for (int c = 0; c < 10; ++c) {
             const uint32_t d = (c <= 8 && ((born >> c) & 1u)) ? 0xFFFFFFFFu : 0u;
             const uint32_t a =
                 (c >= 1 && (((born | sustain) >> (c - 1)) & 1u)) ? 0xFFFFFFFFu : 0u;
-            leaf[c] = bitop3<kMux>(m, a, d);
+            leaf[c] = c == 0 ? d : c == 9 ? a : bitop3<kMux>(m, a, d);
         }
         const uint32_t l0 = bitop3<kMux>(t0, leaf[1], leaf[0]);
         const uint32_t l1 = bitop3<kMux>(t0, leaf[3], leaf[2]);
