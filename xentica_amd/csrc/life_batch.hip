@@ -198,6 +198,9 @@ void life_batch_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ 
         out_off += row_bytes;
     };
 
+    // highest user priority first, one step lower every two iterations
+    // (lower_priority): the waves of a SIMD finish together
+    __builtin_amdgcn_s_setprio(3);
     LaneRule rule;
     rule.init(static_cast<uint32_t>(r.x), static_cast<uint32_t>(r.y));
     Levels<K> w;
@@ -234,6 +237,7 @@ void life_batch_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ 
         feed_rows<K, P2, true>(w, word_of(x2), row + 2, rule);
         store_row(top_row<K, P2>(w), row + 2 - 2 * K);
     };
+    int prio = 3, iters = 0;
     for (; i + 5 < L; i += 6) {
 #pragma unroll
         for (int r = 0; r < 6; ++r) nxt[r] = load_next();
@@ -241,6 +245,7 @@ void life_batch_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ 
         feed3(cur[3], cur[4], cur[5], i + 3);
 #pragma unroll
         for (int r = 0; r < 6; ++r) cur[r] = nxt[r];
+        if ((++iters & 1) == 0) lower_priority(prio);
     }
     // the last 0 .. 5 rows
 #pragma unroll

@@ -47,6 +47,22 @@ constexpr uint8_t kAndOr2 = bitop3_table([](bool t0, bool e1, bool f) { return (
 constexpr uint8_t kXorAnd = bitop3_table([](bool v, bool u, bool k) { return v != (u && k); });
 constexpr uint8_t kAnd3 = bitop3_table([](bool v, bool u, bool k) { return v && u && k; });
 
+// One step down the wave's user priority (s_setprio takes an immediate),
+// from 3 to 0 and no further.  A SIMD issues from its oldest ready wave
+// first, so of several waves with equal work the oldest finishes long
+// before the youngest, which then runs alone and cannot keep the VALU
+// busy; kernels start every wave at priority 3 and lower it as the wave
+// progresses, so a wave that is ahead yields to the waves behind it.
+__device__ __forceinline__ void lower_priority(int &prio) {
+    --prio;
+    if (prio == 2)
+        __builtin_amdgcn_s_setprio(2);
+    else if (prio == 1)
+        __builtin_amdgcn_s_setprio(1);
+    else if (prio == 0)
+        __builtin_amdgcn_s_setprio(0);
+}
+
 // A row word with its horizontal 3-sums: h0 + 2 h1 = (x-1) + x + (x+1).
 struct HRow {
     uint32_t raw, h0, h1;
