@@ -437,6 +437,14 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
 #if !_CDIRECT
     __shared__ int3 _col[_TX * _TY + 256];
 #endif
+    // A SIMD issues from its oldest ready wave first: workgroups start at
+    // the highest user priority and step down after each quarter of their
+    // share of tiles, so the workgroups of a CU finish together instead of
+    // the youngest running its last tiles alone.
+    __builtin_amdgcn_s_setprio(3);
+    const int _share = (_n_tiles - (int) blockIdx.x + (int) gridDim.x - 1) / (int) gridDim.x;
+    const int _step_at = _share >= 4 ? _share / 4 : 1;
+    int _done = 0, _prio = 3;
     for (int _tile = blockIdx.x; _tile < _n_tiles; _tile += gridDim.x) {{
         const _Inst _in = _insts[_tile_inst[_tile]];
         const int _t = _tile - _in.tile0;
@@ -558,6 +566,15 @@ void fused_bsca(const _word *__restrict__ _src_all, _word *__restrict__ _dst_all
         }}
 #endif
 {render_epilogue}        __syncthreads();
+        if (++_done % _step_at == 0 && _prio > 0) {{
+            --_prio;
+            if (_prio == 2)
+                __builtin_amdgcn_s_setprio(2);
+            else if (_prio == 1)
+                __builtin_amdgcn_s_setprio(1);
+            else
+                __builtin_amdgcn_s_setprio(0);
+        }}
     }}
 }}
 """
