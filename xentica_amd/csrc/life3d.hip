@@ -252,6 +252,12 @@ void life3d_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__
     Rule3<kFast> rule{born, sustain};
     const int pb = g.plane_bytes;
 
+    // highest user priority first, one step lower after each quarter of
+    // the wave's planes: the waves of a SIMD finish together instead of
+    // the oldest first (a SIMD issues from its oldest ready wave)
+    __builtin_amdgcn_s_setprio(3);
+    const int quarter = (z_hi - z_lo + 3) / 4;
+    int prio = 3, next_step = z_lo + quarter;
     Plane<YS> p0, p1, p2;
     load_plane<YS>(g, z_lo - 1, p0);
     load_plane<YS>(g, z_lo, p1);
@@ -263,6 +269,16 @@ void life3d_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__
         emit_plane<YS, kFast>(p1, p2, p0, rule, out, out_off, (z + 1) * pb);
         load_plane<YS>(g, z + 3, p1);
         emit_plane<YS, kFast>(p2, p0, p1, rule, out, out_off, (z + 2) * pb);
+        if (z + 3 >= next_step) {
+            next_step += quarter;
+            --prio;
+            if (prio == 2)
+                __builtin_amdgcn_s_setprio(2);
+            else if (prio == 1)
+                __builtin_amdgcn_s_setprio(1);
+            else if (prio == 0)
+                __builtin_amdgcn_s_setprio(0);
+        }
     }
     if (z < z_hi) {
         load_plane<YS>(g, z + 1, p2);
