@@ -473,10 +473,16 @@ void life_bits_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ d
             for (int r = 0; r < kRows; ++r) nxt[r] = load_next();
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int r = 0; r < kRows; r += 2) feed2(cur[r], cur[r + 1], i + r);
+            for (int r = 0; r < kRows; r += 2) {
+                feed2(cur[r], cur[r + 1], i + r);
+                // the step down after six of the ten rows puts the three
+                // steps of an exchanging strip near a quarter, a half and
+                // three quarters of its work (2 % faster at 16384^2 than
+                // at the end of the iteration)
+                if (r == 4) lower_priority(prio);
+            }
 #pragma unroll
             for (int r = 0; r < kRows; ++r) cur[r] = nxt[r];
-            lower_priority(prio);
         }
         // the last 0 .. kRows - 1 rows
 #pragma unroll
