@@ -18,12 +18,18 @@
 //     interior row segments of it share a wave, one group of rem + 2 lanes
 //     each, so a 16384-cell row (512 words = 8 x 62 + 16) runs on 8 1/3
 //     waves per row strip instead of 9;
-//   * each lane keeps R = OUT + 2K rows of its column in VGPRs and applies
-//     K steps in registers (temporal blocking: K steps per HBM round trip);
+//   * each lane streams the rows of its column through a K-level row
+//     pipeline in VGPRs (temporal blocking: K steps per HBM round trip);
+//     the strips of a workgroup hand their last rows of each level to the
+//     next strip through LDS instead of computing halo rows;
 //   * vertical neighbour sums come from registers, horizontal ones from the
 //     adjacent lanes through DPP wave shifts plus a funnel shift;
 //   * the neighbour count is a bit-sliced carry-save adder over 32 cells at
-//     once; for B3/S23 the rule collapses to two 3-input logic ops.
+//     once; B3/S23 takes rows in pairs (a pair sum shared by two output
+//     rows, then four 3-input logic ops per output), other rules three rows
+//     and a mux tree over the count;
+//   * waves step their priority down as they progress, so the waves of a
+//     SIMD finish together.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
