@@ -39,11 +39,6 @@ extern "C" {
 int xen_life_bits_launch(const void *src, void *dst, int width_words, int height,
                          uint32_t born, uint32_t sustain, int depth, int strips,
                          xen_stream stream);
-/* Load every depth and rule variant of the kernel on the current device
- * before the first step, as the reference loads its module when a model
- * is built (xentica/core/base.py:538-541): the runtime otherwise resolves
- * each variant at its first launch (about 10 us each on MI355X). */
-int xen_life_bits_prepare(void);
 /* Advance `steps` steps ping-ponging between buf_a (holding the state) and
  * buf_b; *result_in_b tells which buffer holds the result. */
 int xen_life_bits_run(void *buf_a, void *buf_b, int width_words, int height,

@@ -735,41 +735,9 @@ int auto_strips(int W, int H, int depth) {
     return last_s;
 }
 
-// One launch of every depth variant of a rule on a 1 x 64-word field.
-template <bool kFast, int... Ks>
-int prepare_all(const uint32_t *src, uint32_t *dst, uint32_t born, uint32_t sustain,
-                std::integer_sequence<int, Ks...>) {
-    int rc = 0;
-    ((rc = rc ? rc
-              : launch_k<Ks + 1, kFast>(src, dst, 1, 64, make_plan(1, 64, Ks + 1, 1), born,
-                                        sustain, nullptr)),
-     ...);
-    return rc;
-}
-
 }  // namespace
 
 extern "C" {
-
-int xen_life_bits_prepare(void) {
-    // the runtime finishes loading a kernel at its first launch (resolving
-    // it without a launch does not), so each variant runs once on a tiny
-    // field here instead of inside the caller's first timed steps
-    using Depths = std::make_integer_sequence<int, 8>;
-    constexpr size_t kBytes = 64 * 4;
-    void *a = nullptr, *b = nullptr;
-    if (int rc = xen::check(hipMalloc(&a, kBytes), "hipMalloc")) return rc;
-    int rc = xen::check(hipMalloc(&b, kBytes), "hipMalloc");
-    if (!rc) rc = xen::check(hipMemset(a, 0, kBytes), "hipMemset");
-    auto src = static_cast<const uint32_t *>(a);
-    auto dst = static_cast<uint32_t *>(b);
-    if (!rc) rc = prepare_all<true>(src, dst, 0x8u, 0xCu, Depths{});
-    if (!rc) rc = prepare_all<false>(src, dst, 0x48u, 0xCu, Depths{});
-    if (!rc) rc = xen::check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    hipFree(a);
-    if (b) hipFree(b);
-    return rc;
-}
 
 int xen_life_bits_launch(const void *src, void *dst, int width_words, int height,
                          uint32_t born, uint32_t sustain, int depth, int strips,

@@ -246,9 +246,6 @@ class BitsEngine:
                                  sustain)
                 self._cur = 1 - self._cur
         elif self.dims == 2:
-            # every depth variant loaded with the first launch, not one at
-            # a time at their first use (a short run would pay for each)
-            kernels.life_bits_prepare()
             width, height = self.model.size
             in_b = kernels.life_bits_run(
                 bufs[self._cur].ptr, bufs[1 - self._cur].ptr, width // 32,

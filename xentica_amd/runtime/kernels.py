@@ -25,7 +25,6 @@ _i = ctypes.c_int
 
 _SIGNATURES = {
     'xen_life_bits_launch': [_vp, _vp, _i, _i, _u32, _u32, _i, _i, _vp],
-    'xen_life_bits_prepare': [],
     'xen_life_bits_run': [_vp, _vp, _i, _i, _u32, _u32, _ll, _i, _i,
                           ctypes.POINTER(_i), _vp],
     'xen_life3d_bits_step': [_vp, _vp, _i, _i, _i, _u32, _u32, _i, _vp],
@@ -66,18 +65,6 @@ def life_bits_launch(src, dst, width_words, height, born, sustain, depth,
     hip.check(_lib().xen_life_bits_launch(
         src, dst, width_words, height, born, sustain, depth, strips,
         _s(stream)), "xen_life_bits_launch")
-
-
-def life_bits_prepare():
-    """Load every variant of the 2-D bit-sliced kernel on the current
-    device (once per process; later calls return at once)."""
-    global _BITS_PREPARED
-    if not _BITS_PREPARED:
-        hip.check(_lib().xen_life_bits_prepare(), "xen_life_bits_prepare")
-        _BITS_PREPARED = True
-
-
-_BITS_PREPARED = False
 
 
 def life_bits_run(buf_a, buf_b, width_words, height, born, sustain, steps,

@@ -44,7 +44,6 @@ class LifeBitsField:
         nbytes = self.n_cells // 8
         self._bufs = [hip.Buffer(nbytes), hip.Buffer(nbytes)]
         self._cur = 0
-        kernels.life_bits_prepare()
 
     @property
     def state_ptr(self):
